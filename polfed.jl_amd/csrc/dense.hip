@@ -1,0 +1,414 @@
+// Dense FP64 block kernels of the block-Lanczos step: tall-skinny Gram
+// products A^T B and updates B = beta B + alpha A G on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64), Cholesky-QR, small rotations and residual norms.
+//
+// MFMA f64 16x16x4 operand layout used here (lane l of a 64-wide wave):
+//   A fragment (16 x 4): lane l holds A[l % 16][l / 16]
+//   B fragment (4 x 16): lane l holds B[l / 16][l % 16]
+//   C/D (16 x 16):       lane l, register v holds D[l/16 + 4*v][l % 16]
+// (output layout measured on gfx950 with an in-bounds probe of known inputs).
+// The contraction index k only has to agree between the A and B fragments, so
+// the Gram kernel assigns k-slot l/16 to rows 4*(l/16)+q of a 16-row slab for
+// the q-th of four MFMAs: each lane then loads 4 consecutive rows (32 bytes) of
+// one column and 4 lanes cover a full 128-byte line.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pf_internal.h"
+
+namespace pf {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mfma16x16x4(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int kGramRowBlocks = 128;  // row partitions of the Gram reduction
+constexpr int kMaxN = 64;            // widest B block of the Gram/update kernels
+
+// Gram partials: part[(rb * m + i) * n + j] = sum over rb's slabs of A[:, i] B[:, j].
+// Grid: (kGramRowBlocks, ceil(m/64)); block: 4 waves, wave w owns A columns
+// i0 = 64*blockIdx.y + 16*w .. +15 and all n columns of B (NT <= 4 tiles).
+template <int NT>
+__global__ __launch_bounds__(256) void gram_kernel(const double* __restrict__ A, int64_t lda, int m,
+                                                   const double* __restrict__ B, int64_t ldb, int n,
+                                                   int64_t D, double* __restrict__ part) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i0 = blockIdx.y * 64 + w * 16;
+  const int li = lane & 15, kq = lane >> 4;
+  const int ia = i0 + li;
+  d4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = (d4){0.0, 0.0, 0.0, 0.0};
+  const int64_t nslabs = (D + 15) / 16;
+  const bool full = (D % 16) == 0;
+  for (int64_t sl = blockIdx.x; sl < nslabs; sl += gridDim.x) {
+    const int64_t r0 = sl * 16 + 4 * kq;  // this lane's 4 rows
+    double av[4];
+    if (ia < m) {
+      const double* pa = A + (int64_t)ia * lda + r0;
+      if (full || r0 + 3 < D) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) av[q] = pa[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) av[q] = (r0 + q < D) ? pa[q] : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) av[q] = 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int jb = t * 16 + li;
+      double bv[4];
+      if (jb < n) {
+        const double* pb = B + (int64_t)jb * ldb + r0;
+        if (full || r0 + 3 < D) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) bv[q] = pb[q];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) bv[q] = (r0 + q < D) ? pb[q] : 0.0;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bv[q] = 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[t] = mfma16x16x4(av[q], bv[q], acc[t]);
+    }
+  }
+  // D-fragment: row (A column) i0 + kq + 4*v, column (B column) 16*t + li
+  double* out = part + (int64_t)blockIdx.x * m * n;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int j = t * 16 + li;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + kq + 4 * v;
+      if (i < m && j < n) out[(int64_t)i * n + j] = acc[t][v];
+    }
+  }
+}
+
+// G[i + j*ldg] = sum_rb part[(rb*m + i)*n + j]  (fixed order: deterministic)
+__global__ void gram_reduce_kernel(const double* __restrict__ part, int nrb, int m, int n,
+                                   double* __restrict__ G, int ldg) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)m * n) return;
+  const int i = (int)(idx / n), j = (int)(idx % n);
+  double t = 0.0;
+  for (int rb = 0; rb < nrb; ++rb) t += part[((int64_t)rb * m + i) * n + j];
+  G[i + (int64_t)j * ldg] = t;
+}
+
+// B = beta*B + alpha*A*G.  Block: 4 waves, 64 rows (wave w: rows r0+16w..+15),
+// all n output columns (NT tiles of 16).  Contraction over the m columns of A
+// in steps of 4: A fragment lane l -> A[r0 + l%16, k + l/16] (16 consecutive
+// rows of one column), G fragment lane l -> G[k + l/16, 16t + l%16].
+template <int NT>
+__global__ __launch_bounds__(256) void update_kernel(const double* __restrict__ A, int64_t lda,
+                                                     int m, const double* __restrict__ G, int ldg,
+                                                     double* __restrict__ B, int64_t ldb, int n,
+                                                     int64_t D, double alpha, double beta) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t r0 = (int64_t)blockIdx.x * 64 + w * 16;
+  const int li = lane & 15, kq = lane >> 4;
+  const int64_t ra = r0 + li;
+  const bool rok = ra < D;
+  d4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = (d4){0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < m; k += 4) {
+    const int ka = k + kq;
+    const double a = (rok && ka < m) ? A[ra + (int64_t)ka * lda] : 0.0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int j = t * 16 + li;
+      const double g = (ka < m && j < n) ? G[ka + (int64_t)j * ldg] : 0.0;
+      acc[t] = mfma16x16x4(a, g, acc[t]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int j = t * 16 + li;
+    if (j >= n) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t r = r0 + kq + 4 * v;
+      if (r < D) {
+        double* pb = B + r + (int64_t)j * ldb;
+        const double old = (beta == 0.0) ? 0.0 : beta * *pb;
+        *pb = old + alpha * acc[t][v];
+      }
+    }
+  }
+}
+
+// W[r, :] = W[r, :] * M for an n x n matrix M (column-major), one row per thread.
+__global__ __launch_bounds__(256) void rotate_kernel(double* __restrict__ W, int64_t ldw, int64_t D,
+                                                     int n, const double* __restrict__ M) {
+  __shared__ double sM[kMaxN * kMaxN];
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) sM[i] = M[i];
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= D) return;
+  double row[kMaxN];
+#pragma unroll 8
+  for (int i = 0; i < n; ++i) row[i] = W[r + (int64_t)i * ldw];
+  for (int j = 0; j < n; ++j) {
+    double t = 0.0;
+    for (int i = 0; i < n; ++i) t += row[i] * sM[i + j * n];
+    W[r + (int64_t)j * ldw] = t;
+  }
+}
+
+// Column partials of <A_j, B_j> (mode 0) or ||A_j - e_j B_j||^2 (mode 1).
+__global__ __launch_bounds__(256) void coldot_kernel(const double* __restrict__ A, int64_t lda,
+                                                     const double* __restrict__ B, int64_t ldb,
+                                                     int64_t D, const double* __restrict__ e,
+                                                     int mode, double* __restrict__ part) {
+  __shared__ double red[4];
+  const int c = blockIdx.y;
+  double acc = 0.0;
+  const double ec = mode ? e[c] : 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < D;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const double a = A[r + (int64_t)c * lda], b = B[r + (int64_t)c * ldb];
+    if (mode) {
+      const double d = a - ec * b;
+      acc += d * d;
+    } else {
+      acc += a * b;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[(int64_t)c * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void copy_block_kernel(const double* __restrict__ src, int64_t lds, double* __restrict__ dst,
+                                  int64_t ldd, int64_t D) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r < D) dst[r + (int64_t)c * ldd] = src[r + (int64_t)c * lds];
+}
+
+// ---------------------------------------------------------------- drivers
+void gram(pf_handle* h, const double* A, int64_t lda, int m, const double* B, int64_t ldb, int n,
+          int64_t D, double* G, int ldg, hipStream_t st) {
+  PF_REQUIRE(m >= 1 && n >= 1 && n <= kMaxN, "gram: 1 <= n <= 64, m >= 1");
+  PF_REQUIRE(D >= 1 && lda >= D && ldb >= D && ldg >= m, "gram: bad leading dimensions");
+  const int nrb = (int)std::min<int64_t>(kGramRowBlocks, (D + 15) / 16);
+  double* part = reinterpret_cast<double*>(h->partials.get((size_t)nrb * m * n * sizeof(double)));
+  dim3 grid(nrb, (m + 63) / 64), block(256);
+  const int NT = (n + 15) / 16;
+  switch (NT) {
+    case 1: hipLaunchKernelGGL(gram_kernel<1>, grid, block, 0, st, A, lda, m, B, ldb, n, D, part); break;
+    case 2: hipLaunchKernelGGL(gram_kernel<2>, grid, block, 0, st, A, lda, m, B, ldb, n, D, part); break;
+    case 3: hipLaunchKernelGGL(gram_kernel<3>, grid, block, 0, st, A, lda, m, B, ldb, n, D, part); break;
+    default: hipLaunchKernelGGL(gram_kernel<4>, grid, block, 0, st, A, lda, m, B, ldb, n, D, part); break;
+  }
+  PF_HIP(hipGetLastError());
+  const int64_t mn = (int64_t)m * n;
+  hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, part,
+                     nrb, m, n, G, ldg);
+  PF_HIP(hipGetLastError());
+}
+
+void update(const double* A, int64_t lda, int m, const double* G, int ldg, double* B, int64_t ldb,
+            int n, int64_t D, double alpha, double beta, hipStream_t st) {
+  PF_REQUIRE(m >= 1 && n >= 1 && n <= kMaxN, "update: 1 <= n <= 64, m >= 1");
+  PF_REQUIRE(D >= 1 && lda >= D && ldb >= D && ldg >= m, "update: bad leading dimensions");
+  dim3 grid((unsigned)((D + 63) / 64)), block(256);
+  const int NT = (n + 15) / 16;
+  switch (NT) {
+    case 1: hipLaunchKernelGGL(update_kernel<1>, grid, block, 0, st, A, lda, m, G, ldg, B, ldb, n, D, alpha, beta); break;
+    case 2: hipLaunchKernelGGL(update_kernel<2>, grid, block, 0, st, A, lda, m, G, ldg, B, ldb, n, D, alpha, beta); break;
+    case 3: hipLaunchKernelGGL(update_kernel<3>, grid, block, 0, st, A, lda, m, G, ldg, B, ldb, n, D, alpha, beta); break;
+    default: hipLaunchKernelGGL(update_kernel<4>, grid, block, 0, st, A, lda, m, G, ldg, B, ldb, n, D, alpha, beta); break;
+  }
+  PF_HIP(hipGetLastError());
+}
+
+// per-column reductions returned to the host (fixed order)
+static void coldots_host(pf_handle* h, const double* A, int64_t lda, const double* B, int64_t ldb,
+                         int64_t D, int n, const double* e_dev, int mode, double* out,
+                         hipStream_t st) {
+  const int nblk = (int)std::min<int64_t>(256, (D + 255) / 256);
+  double* part = reinterpret_cast<double*>(h->partials.get((size_t)nblk * n * sizeof(double)));
+  hipLaunchKernelGGL(coldot_kernel, dim3(nblk, n), dim3(256), 0, st, A, lda, B, ldb, D, e_dev, mode,
+                     part);
+  PF_HIP(hipGetLastError());
+  std::vector<double> hp((size_t)nblk * n);
+  PF_HIP(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  PF_HIP(hipStreamSynchronize(st));
+  for (int c = 0; c < n; ++c) {
+    double t = 0.0;
+    for (int b = 0; b < nblk; ++b) t += hp[(size_t)c * nblk + b];
+    out[c] = t;
+  }
+}
+
+// Cholesky G = R^T R of a small SPD matrix (column-major, n x n); returns false
+// when a pivot is not positive.
+static bool cholesky_upper(const std::vector<double>& G, int n, std::vector<double>& R) {
+  R.assign((size_t)n * n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    double d = G[j + (size_t)j * n];
+    for (int k = 0; k < j; ++k) d -= R[k + (size_t)j * n] * R[k + (size_t)j * n];
+    if (!(d > 0.0) || !std::isfinite(d)) return false;
+    const double rjj = std::sqrt(d);
+    R[j + (size_t)j * n] = rjj;
+    for (int i = j + 1; i < n; ++i) {
+      double t = G[j + (size_t)i * n];
+      for (int k = 0; k < j; ++k) t -= R[k + (size_t)j * n] * R[k + (size_t)i * n];
+      R[j + (size_t)i * n] = t / rjj;
+    }
+  }
+  return true;
+}
+
+// inverse of an upper-triangular n x n matrix (column-major)
+static void inv_upper(const std::vector<double>& R, int n, std::vector<double>& Ri) {
+  Ri.assign((size_t)n * n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    Ri[j + (size_t)j * n] = 1.0 / R[j + (size_t)j * n];
+    for (int i = j - 1; i >= 0; --i) {
+      double t = 0.0;
+      for (int k = i + 1; k <= j; ++k) t += R[i + (size_t)k * n] * Ri[k + (size_t)j * n];
+      Ri[i + (size_t)j * n] = -t / R[i + (size_t)i * n];
+    }
+  }
+}
+
+static void rotate(pf_handle* h, double* W, int64_t ldw, int64_t D, int n, const double* M_host,
+                   hipStream_t st) {
+  PF_REQUIRE(n >= 1 && n <= kMaxN, "rotate: 1 <= n <= 64");
+  double* dM = reinterpret_cast<double*>(h->small.get((size_t)n * n * sizeof(double)));
+  PF_HIP(hipMemcpyAsync(dM, M_host, (size_t)n * n * sizeof(double), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(rotate_kernel, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, st, W, ldw, D,
+                     n, dM);
+  PF_HIP(hipGetLastError());
+  // the host matrix may be freed by the caller after return
+  PF_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+extern "C" int pf_gram(pf_handle* h, const double* A, int64_t lda, int32_t m, const double* B,
+                       int64_t ldb, int32_t n, int64_t D, double* G, int32_t ldg, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(h && A && B && G, "gram: null argument");
+    gram(h, A, lda, m, B, ldb, n, D, G, ldg, S(stream));
+  });
+}
+
+extern "C" int pf_gemm_update(pf_handle* h, const double* A, int64_t lda, int32_t m,
+                              const double* G, int32_t ldg, double* B, int64_t ldb, int32_t n,
+                              int64_t D, double alpha, double beta, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(h && A && G && B, "update: null argument");
+    PF_REQUIRE((const void*)A != (const void*)B, "update: B must not alias A");
+    update(A, lda, m, G, ldg, B, ldb, n, D, alpha, beta, S(stream));
+  });
+}
+
+extern "C" int pf_reorth(pf_handle* h, const double* V, int64_t ldv, int32_t k, double* W,
+                         int64_t ldw, int32_t s, int64_t D, int32_t passes, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(h && V && W, "reorth: null argument");
+    PF_REQUIRE(passes >= 1 && passes <= 4, "reorth: 1 <= passes <= 4");
+    if (k <= 0) return;
+    hipStream_t st = S(stream);
+    double* G = reinterpret_cast<double*>(h->vecs.get((size_t)k * s * sizeof(double)));
+    for (int p = 0; p < passes; ++p) {
+      gram(h, V, ldv, k, W, ldw, s, D, G, k, st);
+      update(V, ldv, k, G, k, W, ldw, s, D, -1.0, 1.0, st);
+    }
+  });
+}
+
+extern "C" int pf_block_qr(pf_handle* h, double* W, int64_t ldw, int64_t D, int32_t s,
+                           double* R_host, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(h && W && R_host, "qr: null argument");
+    PF_REQUIRE(s >= 1 && s <= kMaxN && D >= s && ldw >= D, "qr: bad shape");
+    hipStream_t st = S(stream);
+    // keep a copy to restore W when Cholesky breaks down
+    double* save = reinterpret_cast<double*>(
+        h->vecs.get((size_t)D * s * sizeof(double) + (size_t)s * s * sizeof(double)));
+    double* dG = save + (size_t)D * s;
+    hipLaunchKernelGGL(copy_block_kernel, dim3((unsigned)((D + 255) / 256), s), dim3(256), 0, st, W,
+                       ldw, save, D, D);
+    PF_HIP(hipGetLastError());
+    std::vector<double> G((size_t)s * s), R1, R2, Ri, R((size_t)s * s, 0.0);
+    for (int pass = 0; pass < 2; ++pass) {
+      gram(h, W, ldw, s, W, ldw, s, D, dG, s, st);
+      PF_HIP(hipMemcpyAsync(G.data(), dG, G.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+      PF_HIP(hipStreamSynchronize(st));
+      std::vector<double>& Rp = pass == 0 ? R1 : R2;
+      if (!cholesky_upper(G, s, Rp)) {
+        hipLaunchKernelGGL(copy_block_kernel, dim3((unsigned)((D + 255) / 256), s), dim3(256), 0,
+                           st, save, D, W, ldw, D);
+        PF_HIP(hipGetLastError());
+        PF_HIP(hipStreamSynchronize(st));
+        throw Error{PF_ERR_NUMERIC, "qr: Gram matrix not positive definite (rank-deficient block)"};
+      }
+      inv_upper(Rp, s, Ri);
+      rotate(h, W, ldw, D, s, Ri.data(), st);
+    }
+    // R = R2 * R1 (both upper triangular)
+    for (int j = 0; j < s; ++j)
+      for (int i = 0; i <= j; ++i) {
+        double t = 0.0;
+        for (int k = i; k <= j; ++k) t += R2[i + (size_t)k * s] * R1[k + (size_t)j * s];
+        R[i + (size_t)j * s] = t;
+      }
+    std::memcpy(R_host, R.data(), R.size() * sizeof(double));
+  });
+}
+
+extern "C" int pf_rotate(pf_handle* h, double* W, int64_t ldw, int64_t D, int32_t n,
+                         const double* M_host, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(h && W && M_host && ldw >= D && D >= 1, "rotate: bad argument");
+    rotate(h, W, ldw, D, n, M_host, S(stream));
+  });
+}
+
+extern "C" int pf_coldots(pf_handle* h, const double* A, int64_t lda, const double* B, int64_t ldb,
+                          int64_t D, int32_t n, double* out_host, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(h && A && B && out_host && n >= 1 && n <= 65535 && D >= 1 && lda >= D && ldb >= D,
+               "coldots: bad argument");
+    coldots_host(h, A, lda, B, ldb, D, n, nullptr, 0, out_host, S(stream));
+  });
+}
+
+extern "C" int pf_residuals(pf_handle* h, pf_op* op, const double* psi, int64_t ld, int32_t nev,
+                            double* E_host, double* res_host, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(h && op && psi && E_host && res_host, "residuals: null argument");
+    PF_REQUIRE(nev >= 1 && nev <= 65535 && ld >= op->D, "residuals: bad shape");
+    hipStream_t st = S(stream);
+    const int64_t D = op->D;
+    // H psi with the same leading dimension as psi (the apply kernels share offsets)
+    double* Hpsi = reinterpret_cast<double*>(
+        h->vecs.get((size_t)ld * nev * sizeof(double) + (size_t)nev * sizeof(double)));
+    double* dE = Hpsi + (size_t)ld * nev;
+    int rc = pf_apply(op, PF_F64, 0, psi, ld, Hpsi, ld, nev, stream);
+    if (rc != PF_OK) throw Error{rc, pf_last_error()};
+    coldots_host(h, psi, ld, Hpsi, ld, D, nev, nullptr, 0, E_host, st);
+    PF_HIP(hipMemcpyAsync(dE, E_host, nev * sizeof(double), hipMemcpyHostToDevice, st));
+    coldots_host(h, Hpsi, ld, psi, ld, D, nev, dE, 1, res_host, st);
+    for (int i = 0; i < nev; ++i) res_host[i] = std::sqrt(res_host[i]);
+  });
+}

@@ -1,0 +1,880 @@
+// Matrix-free Hamiltonian kernels, fused Clenshaw steps, filter driver and KPM.
+//
+// Layouts.  The C ABI takes column-major D x s blocks (Julia's layout).  The
+// filter and KPM drivers repack their input once into a row-interleaved block
+// (the s values of a row contiguous), run every recurrence step on that layout
+// -- a gathered neighbour row is then one contiguous s*sizeof(T) segment loaded
+// by G = s/V lanes with V-wide vector loads -- and write the result back
+// column-major in the last (fused) step.  The hook entry points
+// (pf_apply, pf_clenshaw_*) work directly on the caller's column-major blocks.
+//
+// Per row the spin-chain kernel reads the basis state (4 B) and the diagonal
+// (8 B), then for every antiparallel bond gathers the neighbour row at the
+// combinadic rank r + delta (pf_rank.h); no index table is stored.  The Quantum
+// Sun kernel needs no table at all: its full-space basis makes every neighbour
+// an XOR of the row index.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pf_internal.h"
+
+namespace pf {
+
+// ------------------------------------------------------------ vector types
+template <typename T, int V> struct VT;
+template <> struct VT<double, 1> { typedef double t; };
+template <> struct VT<double, 2> { typedef double t __attribute__((ext_vector_type(2))); };
+template <> struct VT<double, 4> { typedef double t __attribute__((ext_vector_type(4))); };
+template <> struct VT<float, 1> { typedef float t; };
+template <> struct VT<float, 2> { typedef float t __attribute__((ext_vector_type(2))); };
+template <> struct VT<float, 4> { typedef float t __attribute__((ext_vector_type(4))); };
+template <> struct VT<float, 8> { typedef float t __attribute__((ext_vector_type(8))); };
+
+template <typename T, int V>
+__device__ __forceinline__ typename VT<T, V>::t vload(const T* p) {
+  return *reinterpret_cast<const typename VT<T, V>::t*>(p);
+}
+template <typename T, int V>
+__device__ __forceinline__ void vstore(T* p, typename VT<T, V>::t v) {
+  *reinterpret_cast<typename VT<T, V>::t*>(p) = v;
+}
+template <typename T, int V>
+__device__ __forceinline__ T vget(typename VT<T, V>::t v, int i) {
+  if constexpr (V == 1) {
+    return v;
+  } else {
+    return v[i];
+  }
+}
+
+// Block-wide sum of one double per thread (256 threads), result in thread 0.
+__device__ __forceinline__ double block_sum_256(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+  return t;
+}
+
+// Shared epilogue: out = cx*X + ch*h + cb*b3, store, optional KPM dot.
+template <typename T, int V>
+__device__ __forceinline__ void step_epilogue(int flags, int64_t row, int64_t rs, int64_t cofs,
+                                              int out_col0, T cx, T ch, T cb,
+                                              typename VT<T, V>::t h, const T* __restrict__ X,
+                                              const T* __restrict__ B3, void* __restrict__ OUT,
+                                              int64_t out_ld, const T* __restrict__ Pr,
+                                              double& dacc) {
+  typedef typename VT<T, V>::t vt;
+  const int64_t off = row * rs + cofs;
+  vt o = (vt)(T)0;
+  if (flags & kHasX) o = cx * vload<T, V>(X + off);
+  if (flags & kHasH) o += ch * h;
+  if (flags & kHasB3) o += cb * vload<T, V>(B3 + off);
+  if (flags & kOutCM) {
+    double* Y = reinterpret_cast<double*>(OUT);
+#pragma unroll
+    for (int v = 0; v < V; ++v) Y[row + (int64_t)(out_col0 + v) * out_ld] = (double)vget<T, V>(o, v);
+  } else {
+    vstore<T, V>(reinterpret_cast<T*>(OUT) + off, o);
+  }
+  if (flags & kDot) {
+    const vt p = vload<T, V>(Pr + off);
+#pragma unroll
+    for (int v = 0; v < V; ++v) dacc += (double)vget<T, V>(p, v) * (double)vget<T, V>(o, v);
+  }
+}
+
+// ------------------------------------------------------ spin-chain kernel
+// out = cx*X + ch*((H - shift)*scale) b2 + cb*b3 on rows of the Lup sector.
+template <typename T, int V>
+__global__ __launch_bounds__(256) void chain_step_kernel(
+    const ChainParams P, const uint32_t* __restrict__ states, const uint32_t* __restrict__ binom,
+    const double* __restrict__ diag, const Affine af, const int flags, const T cx, const T ch,
+    const T cb, const T* __restrict__ X, const T* __restrict__ B2, const T* __restrict__ B3,
+    void* __restrict__ OUT, const Layout lay, const int64_t out_ld, double* __restrict__ dot_part,
+    const T* __restrict__ Pr, const int64_t ngroups) {
+  typedef typename VT<T, V>::t vt;
+  __shared__ uint32_t sC[32 * kRankT];
+  __shared__ double red[4];
+  for (int i = threadIdx.x; i < 32 * kRankT; i += blockDim.x) sC[i] = binom[i];
+  __syncthreads();
+
+  const int rloc = threadIdx.x / lay.G;
+  const int g = threadIdx.x - rloc * lay.G;
+  const bool lane_ok = rloc < lay.rows_per_block;
+  const int64_t cofs = (int64_t)blockIdx.y * lay.cstride + (int64_t)g * V;
+  const int out_col0 = (int)blockIdx.y + g * V;
+  double dacc = 0.0;
+
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int64_t row = grp * lay.rows_per_block + rloc;
+    if (!lane_ok || row >= P.D) continue;
+    vt h = (vt)(T)0;
+    if (flags & kHasH) {
+      const uint32_t st = states[row];
+      const uint32_t r32 = (uint32_t)row;
+      vt tot = (vt)(T)0;
+      for (int c = 0; c < P.ncls; ++c) {
+        vt acc = (vt)(T)0;
+        for (int b = P.cls_begin[c]; b < P.cls_begin[c + 1]; ++b) {
+          const int lo = P.lo[b], hi = P.hi[b];
+          if (((st >> lo) ^ (st >> hi)) & 1u) {
+            const uint32_t r2 = neighbor_rank(st, r32, lo, hi, sC);
+            acc += vload<T, V>(B2 + (int64_t)r2 * lay.rs + cofs);
+          }
+        }
+        tot += (T)(P.cval[c] * af.scale) * acc;
+      }
+      const T d = (T)((diag[row] - af.shift) * af.scale);
+      h = d * vload<T, V>(B2 + row * lay.rs + cofs) + tot;
+    }
+    step_epilogue<T, V>(flags, row, lay.rs, cofs, out_col0, cx, ch, cb, h, X, B3, OUT, out_ld, Pr,
+                        dacc);
+  }
+  if (flags & kDot) {
+    const double t = block_sum_256(dacc, red);
+    if (threadIdx.x == 0) dot_part[blockIdx.x] = t;
+  }
+}
+
+// ------------------------------------------------------ Quantum Sun kernel
+template <typename T, int V>
+__global__ __launch_bounds__(256) void qsun_step_kernel(
+    const QsunParams P, const Affine af, const int flags, const T cx, const T ch, const T cb,
+    const T* __restrict__ X, const T* __restrict__ B2, const T* __restrict__ B3,
+    void* __restrict__ OUT, const Layout lay, const int64_t out_ld, double* __restrict__ dot_part,
+    const T* __restrict__ Pr, const int64_t ngroups) {
+  typedef typename VT<T, V>::t vt;
+  const int dg = 1 << P.Lg;
+  __shared__ T sG[(1 << kMaxGrain) * (1 << kMaxGrain)];
+  __shared__ double red[4];
+  for (int i = threadIdx.x; i < dg * dg; i += blockDim.x) sG[i] = (T)(P.G[i] * af.scale);
+  __syncthreads();
+
+  const int rloc = threadIdx.x / lay.G;
+  const int g = threadIdx.x - rloc * lay.G;
+  const bool lane_ok = rloc < lay.rows_per_block;
+  const int64_t cofs = (int64_t)blockIdx.y * lay.cstride + (int64_t)g * V;
+  const int out_col0 = (int)blockIdx.y + g * V;
+  double dacc = 0.0;
+
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int64_t row = grp * lay.rows_per_block + rloc;
+    if (!lane_ok || row >= P.D) continue;
+    vt h = (vt)(T)0;
+    if (flags & kHasH) {
+      const uint32_t r = (uint32_t)row;
+      const int a = (int)(r & (uint32_t)(dg - 1));
+      const uint32_t base = r & ~(uint32_t)(dg - 1);
+      // diagonal in the reference order: outer fields, then the grain diagonal
+      double dd = 0.0;
+      for (int j = 0; j < P.Lloc; ++j) dd += P.h[j] * (((r >> (P.Lg + j)) & 1u) ? 0.5 : -0.5);
+      dd += P.G[a * dg + a];
+      vt acc = (vt)(T)0;
+      for (int b = 0; b < dg; ++b) {
+        if (b == a) continue;
+        acc += sG[a * dg + b] * vload<T, V>(B2 + (int64_t)(base | (uint32_t)b) * lay.rs + cofs);
+      }
+      for (int j = 0; j < P.Lloc; ++j)
+        acc += (T)(P.coup[j] * af.scale) * vload<T, V>(B2 + (int64_t)(r ^ P.flip[j]) * lay.rs + cofs);
+      const T d = (T)((dd - af.shift) * af.scale);
+      h = d * vload<T, V>(B2 + row * lay.rs + cofs) + acc;
+    }
+    step_epilogue<T, V>(flags, row, lay.rs, cofs, out_col0, cx, ch, cb, h, X, B3, OUT, out_ld, Pr,
+                        dacc);
+  }
+  if (flags & kDot) {
+    const double t = block_sum_256(dacc, red);
+    if (threadIdx.x == 0) dot_part[blockIdx.x] = t;
+  }
+}
+
+// ------------------------------------------------------------ launching
+Layout interleaved_layout(int dtype, int s) {
+  Layout l{};
+  int V = 1;
+  if (dtype == PF_F64) {
+    V = (s % 2 == 0) ? 2 : 1;
+  } else {
+    V = (s % 4 == 0) ? 4 : (s % 2 == 0 ? 2 : 1);
+  }
+  l.V = V;
+  l.G = s / V;
+  l.rs = s;
+  l.cstride = 0;
+  l.rows_per_block = 256 / l.G;
+  return l;
+}
+
+Layout colmajor_layout(int64_t ld) {
+  Layout l{};
+  l.V = 1;
+  l.G = 1;
+  l.rs = 1;
+  l.cstride = ld;
+  l.rows_per_block = 256;
+  return l;
+}
+
+template <typename T, int V>
+static void launch_typed(pf_op* op, int flags, const Layout& lay, int ncols_grid, Affine af,
+                         double cx, double ch, double cb, const void* X, const void* B2,
+                         const void* B3, void* out, int64_t out_ld, hipStream_t st,
+                         double* dot_part, const void* P, int dot_blocks) {
+  const int64_t ngroups = (op->D + lay.rows_per_block - 1) / lay.rows_per_block;
+  int64_t gx = ngroups;
+  if (flags & kDot) gx = std::min<int64_t>(ngroups, dot_blocks);
+  PF_REQUIRE(gx < (int64_t)1 << 31, "grid too large");
+  dim3 grid((unsigned)gx, (unsigned)ncols_grid), block(256);
+  const T* Xt = reinterpret_cast<const T*>(X);
+  const T* B2t = reinterpret_cast<const T*>(B2);
+  const T* B3t = reinterpret_cast<const T*>(B3);
+  const T* Pt = reinterpret_cast<const T*>(P);
+  if (op->kind == kChain) {
+    hipLaunchKernelGGL((chain_step_kernel<T, V>), grid, block, 0, st, op->chain, op->states,
+                       op->binom, op->diag, af, flags, (T)cx, (T)ch, (T)cb, Xt, B2t, B3t, out,
+                       lay, out_ld, dot_part, Pt, ngroups);
+  } else {
+    hipLaunchKernelGGL((qsun_step_kernel<T, V>), grid, block, 0, st, op->qsun, af, flags, (T)cx,
+                       (T)ch, (T)cb, Xt, B2t, B3t, out, lay, out_ld, dot_part, Pt, ngroups);
+  }
+  PF_HIP(hipGetLastError());
+}
+
+void launch_step(pf_op* op, int dtype, int flags, const Layout& lay, int ncols, Affine af,
+                 double cx, double ch, double cb, const void* X, const void* B2, const void* B3,
+                 void* out, int64_t out_ld, hipStream_t st, double* dot_part, const void* P,
+                 int dot_blocks) {
+  const int ny = lay.cstride ? ncols : 1;
+  if (dtype == PF_F64) {
+    switch (lay.V) {
+      case 1: launch_typed<double, 1>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
+      case 2: launch_typed<double, 2>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
+      case 4: launch_typed<double, 4>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
+      default: throw Error{PF_ERR_UNSUPPORTED, "unsupported vector width"};
+    }
+  } else if (dtype == PF_F32) {
+    switch (lay.V) {
+      case 1: launch_typed<float, 1>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
+      case 2: launch_typed<float, 2>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
+      case 4: launch_typed<float, 4>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
+      case 8: launch_typed<float, 8>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
+      default: throw Error{PF_ERR_UNSUPPORTED, "unsupported vector width"};
+    }
+  } else {
+    throw Error{PF_ERR_INVALID, "dtype must be PF_F64 or PF_F32"};
+  }
+}
+
+// ------------------------------------------------------------- pack kernel
+// dst[r*s + c] = (T)(src[r + c*ld] * scale[c]) through a 64-row LDS tile.
+template <typename T>
+__global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ src, int64_t ld,
+                                                   T* __restrict__ dst, int64_t D, int s,
+                                                   const double* __restrict__ scale) {
+  extern __shared__ double tile[];  // 64 x (s+1)
+  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  const int n = 64 * s;
+  for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+    const int c = idx / 64, rr = idx - c * 64;
+    const int64_t r = r0 + rr;
+    double v = 0.0;
+    if (r < D) v = src[r + (int64_t)c * ld] * (scale ? scale[c] : 1.0);
+    tile[rr * (s + 1) + c] = v;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+    const int rr = idx / s, c = idx - rr * s;
+    const int64_t r = r0 + rr;
+    if (r < D) dst[r * s + c] = (T)tile[rr * (s + 1) + c];
+  }
+}
+
+void pack_block(int dtype, const double* src, int64_t ld, void* dst, int64_t D, int s,
+                const double* scale, hipStream_t st) {
+  PF_REQUIRE(s >= 1 && s <= 64, "pack: 1 <= s <= 64");
+  const int64_t nb = (D + 63) / 64;
+  const size_t lds = (size_t)64 * (s + 1) * sizeof(double);
+  if (dtype == PF_F64)
+    hipLaunchKernelGGL(pack_kernel<double>, dim3((unsigned)nb), dim3(256), lds, st, src, ld,
+                       reinterpret_cast<double*>(dst), D, s, scale);
+  else
+    hipLaunchKernelGGL(pack_kernel<float>, dim3((unsigned)nb), dim3(256), lds, st, src, ld,
+                       reinterpret_cast<float*>(dst), D, s, scale);
+  PF_HIP(hipGetLastError());
+}
+
+// --------------------------------------------------------- setup kernels
+// Diagonal of the spin-chain Hamiltonian in the reference summation order
+// (fields for sites 1..L, then J*Delta*zi*zj per bond in input order), with
+// FMA contraction disabled so the value is bit-identical to the reference's
+// H[i, i] (src/Models/SpinHalfChains.jl:285-298).
+__global__ void chain_diag_kernel(const uint32_t* __restrict__ states, int64_t D, int L, int nb,
+                                  const double* __restrict__ fields,
+                                  const int8_t* __restrict__ bi, const int8_t* __restrict__ bj,
+                                  const double* __restrict__ jd, double* __restrict__ diag) {
+#pragma clang fp contract(off)
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= D) return;
+  const uint32_t st = states[r];
+  double d = 0.0;
+  for (int site = 0; site < L; ++site) {
+    const double z = ((st >> site) & 1u) ? 0.5 : -0.5;
+    const double t = fields[site] * z;
+    d = d + t;
+  }
+  for (int b = 0; b < nb; ++b) {
+    const double zi = ((st >> bi[b]) & 1u) ? 0.5 : -0.5;
+    const double zj = ((st >> bj[b]) & 1u) ? 0.5 : -0.5;
+    const double t = ((jd[b] * zi) * zj);
+    d = d + t;
+  }
+  diag[r] = d;
+}
+
+// Connectivity export (parity surface): the bonds of each value class in turn.
+__global__ void chain_conn_kernel(const uint32_t* __restrict__ states,
+                                  const uint32_t* __restrict__ binom, const double* __restrict__ diag,
+                                  int64_t row0, int64_t nrows, int nb, const int8_t* __restrict__ bi,
+                                  const int8_t* __restrict__ bj, const double* __restrict__ halfj,
+                                  int maxper, int64_t* __restrict__ cols, double* __restrict__ vals,
+                                  int32_t* __restrict__ counts, double* __restrict__ dout) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const int64_t row = row0 + t;
+  const uint32_t st = states[row];
+  int cnt = 0;
+  for (int b = 0; b < nb; ++b) {
+    const int i = bi[b], j = bj[b];
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    if (((st >> lo) ^ (st >> hi)) & 1u) {
+      if (cnt < maxper) {
+        cols[t * maxper + cnt] = (int64_t)neighbor_rank(st, (uint32_t)row, lo, hi, binom);
+        vals[t * maxper + cnt] = halfj[b];
+      }
+      ++cnt;
+    }
+  }
+  counts[t] = cnt;
+  dout[t] = diag[row];
+}
+
+__global__ void qsun_conn_kernel(const QsunParams P, int64_t row0, int64_t nrows, int maxper,
+                                 int64_t* __restrict__ cols, double* __restrict__ vals,
+                                 int32_t* __restrict__ counts, double* __restrict__ dout) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const uint32_t r = (uint32_t)(row0 + t);
+  const int dg = 1 << P.Lg;
+  const int a = (int)(r & (uint32_t)(dg - 1));
+  const uint32_t base = r & ~(uint32_t)(dg - 1);
+  int cnt = 0;
+  for (int j = 0; j < P.Lloc; ++j) {
+    if (cnt < maxper) {
+      cols[t * maxper + cnt] = (int64_t)(r ^ P.flip[j]);
+      vals[t * maxper + cnt] = P.coup[j];
+    }
+    ++cnt;
+  }
+  for (int b = 0; b < dg; ++b) {
+    if (b == a) continue;
+    if (cnt < maxper) {
+      cols[t * maxper + cnt] = (int64_t)(base | (uint32_t)b);
+      vals[t * maxper + cnt] = P.G[a * dg + b];
+    }
+    ++cnt;
+  }
+  double dd = 0.0;
+  for (int j = 0; j < P.Lloc; ++j) {
+    const double tz = P.h[j] * (((r >> (P.Lg + j)) & 1u) ? 0.5 : -0.5);
+    dd = dd + tz;
+  }
+  dd = dd + P.G[a * dg + a];
+  counts[t] = cnt;
+  dout[t] = dd;
+}
+
+// KPM / probe helpers ---------------------------------------------------
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__global__ void probes_kernel(int kind, uint64_t seed, int64_t D, int R, double* __restrict__ out,
+                              int64_t ld) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= D) return;
+  const uint64_t key = splitmix64(seed ^ splitmix64(((uint64_t)c << 40) ^ (uint64_t)r));
+  double v;
+  if (kind == PF_PROBE_PHASE) {
+    v = (key >> 63) ? 1.0 : -1.0;
+  } else {
+    // Box-Muller on two 53-bit uniforms in (0, 1]
+    const uint64_t k2 = splitmix64(key);
+    const double u1 = ((double)(key >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)(k2 >> 11) * (1.0 / 9007199254740992.0);
+    v = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  }
+  out[r + (int64_t)c * ld] = v;
+}
+
+// column sums of squares of a column-major block -> partials[c*nblk + blockIdx.x]
+__global__ __launch_bounds__(256) void colnorm2_kernel(const double* __restrict__ A, int64_t ld,
+                                                       int64_t D, double* __restrict__ part) {
+  __shared__ double red[4];
+  const int c = blockIdx.y;
+  double acc = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < D;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const double v = A[r + (int64_t)c * ld];
+    acc += v * v;
+  }
+  const double t = block_sum_256(acc, red);
+  if (threadIdx.x == 0) part[(int64_t)c * gridDim.x + blockIdx.x] = t;
+}
+
+__global__ void inv_sqrt_sum_kernel(const double* __restrict__ part, int nblk, int ncols,
+                                    double* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncols) return;
+  double t = 0.0;
+  for (int i = 0; i < nblk; ++i) t += part[(int64_t)c * nblk + i];
+  out[c] = 1.0 / sqrt(t);
+}
+
+// sum over the blocks of each KPM step: mu[n] += sum_{i < count} part[n*stride + i]
+__global__ void kpm_reduce_kernel(const double* __restrict__ part, int stride, int count, int N,
+                                  double* __restrict__ mu) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double t = 0.0;
+  for (int i = 0; i < count; ++i) t += part[(int64_t)n * stride + i];
+  mu[n] += t;
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+// =========================================================== C ABI: ops
+extern "C" int pf_op_create_spin_chain(pf_handle* h, int32_t L, int32_t Lup, int32_t nbonds,
+                                       const int32_t* site_i, const int32_t* site_j,
+                                       const double* J, const double* Delta, const double* fields,
+                                       pf_op** out) {
+  return guarded([&] {
+    PF_REQUIRE(h && out, "null handle/output");
+    PF_REQUIRE(L >= 2 && L <= 32, "spin chain: 2 <= L <= 32 required");
+    PF_REQUIRE(Lup >= 0 && Lup <= L, "spin chain: 0 <= Lup <= L required");
+    PF_REQUIRE(Lup + 2 <= kRankT, "spin chain: Lup <= 18 required");
+    PF_REQUIRE(nbonds >= 0 && nbonds <= kMaxBonds, "spin chain: at most 128 bonds");
+    PF_REQUIRE(nbonds == 0 || (site_i && site_j && J && Delta), "spin chain: null bond arrays");
+    PF_REQUIRE(fields, "spin chain: null fields");
+    PF_HIP(hipSetDevice(h->device));
+    // basis size
+    uint64_t D = 1;
+    for (int i = 1; i <= Lup; ++i) D = D * (uint64_t)(L - Lup + i) / (uint64_t)i;
+    PF_REQUIRE(D >= 1 && D < ((uint64_t)1 << 32), "spin chain: dimension too large");
+
+    pf_op* op = new pf_op();
+    op->h = h;
+    op->kind = kChain;
+    op->D = (int64_t)D;
+    ChainParams& P = op->chain;
+    P.L = L;
+    P.Lup = Lup;
+    P.D = (int64_t)D;
+    P.nb = nbonds;
+    // value classes: J/2 per bond, grouped in ascending value order
+    std::vector<double> halfj(nbonds), jd(nbonds);
+    std::vector<int8_t> oi(nbonds), oj(nbonds);
+    std::vector<double> classes;
+    for (int b = 0; b < nbonds; ++b) {
+      PF_REQUIRE(site_i[b] >= 1 && site_i[b] <= L && site_j[b] >= 1 && site_j[b] <= L &&
+                     site_i[b] != site_j[b],
+                 "spin chain: bond sites must be distinct and in 1..L");
+      oi[b] = (int8_t)(site_i[b] - 1);
+      oj[b] = (int8_t)(site_j[b] - 1);
+      halfj[b] = J[b] / 2;
+      jd[b] = J[b] * Delta[b];
+      if (halfj[b] != 0.0 &&
+          std::find(classes.begin(), classes.end(), halfj[b]) == classes.end())
+        classes.push_back(halfj[b]);
+    }
+    std::sort(classes.begin(), classes.end());
+    PF_REQUIRE((int)classes.size() <= kMaxClasses, "spin chain: at most 8 distinct couplings");
+    P.ncls = (int)classes.size();
+    int nb = 0;
+    for (int c = 0; c < P.ncls; ++c) {
+      P.cls_begin[c] = nb;
+      P.cval[c] = classes[c];
+      for (int b = 0; b < nbonds; ++b)
+        if (halfj[b] == classes[c]) {
+          P.lo[nb] = std::min(oi[b], oj[b]);
+          P.hi[nb] = std::max(oi[b], oj[b]);
+          ++nb;
+        }
+    }
+    P.cls_begin[P.ncls] = nb;
+
+    // basis (ascending, Gosper's hack) and binomial table
+    std::vector<uint32_t> states(D);
+    if (Lup == 0) {
+      states[0] = 0;
+    } else {
+      uint64_t v = ((uint64_t)1 << Lup) - 1;
+      for (uint64_t r = 0; r < D; ++r) {
+        states[r] = (uint32_t)v;
+        const uint64_t t = v | (v - 1);
+        v = (t + 1) | (((~t & (t + 1)) - 1) >> (__builtin_ctzll(v) + 1));
+      }
+    }
+    std::vector<uint32_t> C(32 * kRankT);
+    fill_binomials(C.data());
+    try {
+      PF_HIP(hipMalloc(&op->states, D * sizeof(uint32_t)));
+      PF_HIP(hipMalloc(&op->binom, C.size() * sizeof(uint32_t)));
+      PF_HIP(hipMalloc(&op->diag, D * sizeof(double)));
+      PF_HIP(hipMemcpy(op->states, states.data(), D * sizeof(uint32_t), hipMemcpyHostToDevice));
+      PF_HIP(hipMemcpy(op->binom, C.data(), C.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      // diagonal on the device (reference order)
+      double* dfields;
+      int8_t* dbi;
+      int8_t* dbj;
+      double* djd;
+      const int nbm = std::max(nbonds, 1);
+      PF_HIP(hipMalloc(&dfields, L * sizeof(double)));
+      PF_HIP(hipMalloc(&dbi, nbm));
+      PF_HIP(hipMalloc(&dbj, nbm));
+      PF_HIP(hipMalloc(&djd, nbm * sizeof(double)));
+      PF_HIP(hipMemcpy(dfields, fields, L * sizeof(double), hipMemcpyHostToDevice));
+      if (nbonds) {
+        PF_HIP(hipMemcpy(dbi, oi.data(), nbonds, hipMemcpyHostToDevice));
+        PF_HIP(hipMemcpy(dbj, oj.data(), nbonds, hipMemcpyHostToDevice));
+        PF_HIP(hipMemcpy(djd, jd.data(), nbonds * sizeof(double), hipMemcpyHostToDevice));
+      }
+      hipLaunchKernelGGL(chain_diag_kernel, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, 0,
+                         op->states, (int64_t)D, L, nbonds, dfields, dbi, dbj, djd, op->diag);
+      PF_HIP(hipGetLastError());
+      PF_HIP(hipDeviceSynchronize());
+      (void)hipFree(dfields);
+      (void)hipFree(dbi);
+      (void)hipFree(dbj);
+      (void)hipFree(djd);
+    } catch (...) {
+      pf_op_destroy(op);
+      throw;
+    }
+    *out = op;
+  });
+}
+
+extern "C" int pf_op_create_qsun(pf_handle* h, int32_t L_grain, int32_t L_loc, double g0,
+                                 const double* H_grain, const int32_t* neighbors,
+                                 const double* couplings, const double* fields, pf_op** out) {
+  return guarded([&] {
+    PF_REQUIRE(h && out, "null handle/output");
+    PF_REQUIRE(L_grain >= 1 && L_grain <= kMaxGrain, "qsun: 1 <= L_grain <= 4");
+    PF_REQUIRE(L_loc >= 1 && L_loc <= kMaxLoc && L_grain + L_loc <= 31, "qsun: bad L_loc");
+    PF_REQUIRE(H_grain && neighbors && couplings && fields, "qsun: null arrays");
+    PF_HIP(hipSetDevice(h->device));
+    pf_op* op = new pf_op();
+    op->h = h;
+    op->kind = kQsun;
+    op->D = (int64_t)1 << (L_grain + L_loc);
+    QsunParams& P = op->qsun;
+    P.Lg = L_grain;
+    P.Lloc = L_loc;
+    P.D = op->D;
+    const int dg = 1 << L_grain;
+    for (int j = 0; j < L_loc; ++j) {
+      if (neighbors[j] < 0 || neighbors[j] >= L_grain) {
+        delete op;
+        throw Error{PF_ERR_INVALID, "qsun: neighbors must be in [0, L_grain)"};
+      }
+      P.flip[j] = (1u << (L_grain + j)) | (1u << neighbors[j]);
+      P.coup[j] = g0 * couplings[j] * 0.5 * 0.5;
+      P.h[j] = fields[j];
+    }
+    // row-major copy of the (column-major) grain matrix: G[a*dg + b] = H[a, b]
+    for (int a = 0; a < dg; ++a)
+      for (int b = 0; b < dg; ++b) P.G[a * dg + b] = H_grain[a + b * dg];
+    *out = op;
+  });
+}
+
+extern "C" int pf_op_destroy(pf_op* op) {
+  if (!op) return PF_OK;
+  if (op->states) (void)hipFree(op->states);
+  if (op->binom) (void)hipFree(op->binom);
+  if (op->diag) (void)hipFree(op->diag);
+  op->fbuf.release();
+  op->kpart.release();
+  delete op;
+  return PF_OK;
+}
+
+extern "C" int64_t pf_op_dim(const pf_op* op) { return op ? op->D : 0; }
+
+extern "C" int pf_op_set_rescale(pf_op* op, double Emin, double Emax) {
+  return guarded([&] {
+    PF_REQUIRE(op, "null operator");
+    PF_REQUIRE(std::isfinite(Emin) && std::isfinite(Emax) && Emax > Emin,
+               "rescale: need finite Emin < Emax");
+    op->Emin = Emin;
+    op->Emax = Emax;
+  });
+}
+
+extern "C" int pf_op_connectivity(pf_op* op, int64_t row0, int64_t nrows, int32_t max_per_row,
+                                  int64_t* cols, double* vals, int32_t* counts, double* diag,
+                                  pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(op, "null operator");
+    PF_REQUIRE(row0 >= 0 && nrows >= 0 && row0 + nrows <= op->D, "connectivity: rows out of range");
+    PF_REQUIRE(max_per_row >= 1 && cols && vals && counts && diag, "connectivity: bad outputs");
+    if (nrows == 0) return;
+    hipStream_t st = S(stream);
+    const unsigned nb = (unsigned)((nrows + 255) / 256);
+    if (op->kind == kChain) {
+      const ChainParams& P = op->chain;
+      // the export walks the class-sorted bond list the step kernels use
+      std::vector<int8_t> bi(P.nb), bj(P.nb);
+      std::vector<double> hv(P.nb);
+      for (int c = 0; c < P.ncls; ++c)
+        for (int b = P.cls_begin[c]; b < P.cls_begin[c + 1]; ++b) {
+          bi[b] = P.lo[b];
+          bj[b] = P.hi[b];
+          hv[b] = P.cval[c];
+        }
+      int8_t *dbi, *dbj;
+      double* dh;
+      const int nbm = std::max(P.nb, 1);
+      PF_HIP(hipMalloc(&dbi, nbm));
+      PF_HIP(hipMalloc(&dbj, nbm));
+      PF_HIP(hipMalloc(&dh, nbm * sizeof(double)));
+      if (P.nb) {
+        PF_HIP(hipMemcpy(dbi, bi.data(), P.nb, hipMemcpyHostToDevice));
+        PF_HIP(hipMemcpy(dbj, bj.data(), P.nb, hipMemcpyHostToDevice));
+        PF_HIP(hipMemcpy(dh, hv.data(), P.nb * sizeof(double), hipMemcpyHostToDevice));
+      }
+      hipLaunchKernelGGL(chain_conn_kernel, dim3(nb), dim3(256), 0, st, op->states, op->binom,
+                         op->diag, row0, nrows, P.nb, dbi, dbj, dh, max_per_row, cols, vals,
+                         counts, diag);
+      PF_HIP(hipGetLastError());
+      PF_HIP(hipStreamSynchronize(st));
+      (void)hipFree(dbi);
+      (void)hipFree(dbj);
+      (void)hipFree(dh);
+    } else {
+      hipLaunchKernelGGL(qsun_conn_kernel, dim3(nb), dim3(256), 0, st, op->qsun, row0, nrows,
+                         max_per_row, cols, vals, counts, diag);
+      PF_HIP(hipGetLastError());
+      PF_HIP(hipStreamSynchronize(st));
+    }
+  });
+}
+
+// =========================================================== C ABI: maps
+extern "C" int pf_apply(pf_op* op, int32_t dtype, int32_t rescaled, const void* X, int64_t ldx,
+                        void* Y, int64_t ldy, int32_t ncols, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(op && X && Y, "apply: null argument");
+    PF_REQUIRE(ncols >= 1 && ncols <= 65535, "apply: 1 <= ncols <= 65535");
+    PF_REQUIRE(ldx == ldy && ldx >= op->D, "apply: X and Y need the same leading dimension >= D");
+    PF_REQUIRE(X != Y, "apply: X and Y must not alias");
+    const Layout lay = colmajor_layout(ldx);
+    launch_step(op, dtype, kHasH, lay, ncols, op->affine(rescaled != 0), 0.0, 1.0, 0.0, nullptr, X,
+                nullptr, Y, 0, S(stream));
+  });
+}
+
+extern "C" int pf_clenshaw_recurrence(pf_op* op, int32_t dtype, void* b1, const void* b2,
+                                      const void* b3, double c, double alpha, double beta,
+                                      const void* X, int64_t ld, int32_t ncols, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(op && b1 && b2 && X, "recurrence: null argument");
+    PF_REQUIRE(ncols >= 1 && ncols <= 65535 && ld >= op->D, "recurrence: bad shape");
+    PF_REQUIRE(b1 != b2, "recurrence: b1 must not alias b2");
+    int flags = kHasX | kHasH;
+    if (beta != 0.0) {
+      PF_REQUIRE(b3 != nullptr, "recurrence: b3 is null but beta != 0");
+      flags |= kHasB3;
+    }
+    launch_step(op, dtype, flags, colmajor_layout(ld), ncols, op->affine(true), c, alpha, beta, X,
+                b2, b3, b1, 0, S(stream));
+  });
+}
+
+extern "C" int pf_clenshaw_finalsum(pf_op* op, int32_t dtype, const void* b2, const void* b3,
+                                    double c0, double beta1, double phi0, double phi1, void* Y,
+                                    const void* X, int64_t ld, int32_t ncols, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(op && b2 && Y && X, "finalsum: null argument");
+    PF_REQUIRE(ncols >= 1 && ncols <= 65535 && ld >= op->D, "finalsum: bad shape");
+    PF_REQUIRE(Y != b2, "finalsum: Y must not alias b2");
+    int flags = kHasX | kHasH;
+    if (beta1 * phi0 != 0.0) {
+      PF_REQUIRE(b3 != nullptr, "finalsum: b3 is null");
+      flags |= kHasB3;
+    }
+    launch_step(op, dtype, flags, colmajor_layout(ld), ncols, op->affine(true), c0 * phi0, phi1,
+                beta1 * phi0, X, b2, b3, Y, 0, S(stream));
+  });
+}
+
+// polynomial_properties (src/Clenshaw/ClenshawMapping.jl:7-12)
+static void poly_props(int type, int k, double& alpha, double& beta_k) {
+  switch (type) {
+    case PF_CHEBYSHEV: alpha = 2.0; beta_k = -1.0; break;
+    case PF_LEGENDRE: alpha = (2.0 * k + 1) / (k + 1); beta_k = -(double)k / (k + 1); break;
+    case PF_HERMITE: alpha = 2.0; beta_k = -2.0 * k; break;
+    case PF_TAYLOR: alpha = 1.0; beta_k = 0.0; break;
+    default: throw Error{PF_ERR_INVALID, "unknown polynomial type"};
+  }
+}
+static void poly_phi(int type, double& phi0, double& phi1) {
+  phi0 = 1.0;
+  phi1 = (type == PF_HERMITE) ? 2.0 : 1.0;
+}
+
+namespace pf {
+// Chunk width for the row-interleaved drivers (columns are independent).
+static int chunk_cols(int dtype) { return dtype == PF_F64 ? 32 : 64; }
+}  // namespace pf
+
+extern "C" int pf_filter(pf_op* op, int32_t dtype, int32_t polytype, const double* coeffs,
+                         int32_t order, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                         int32_t ncols, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(op && coeffs && X && Y, "filter: null argument");
+    PF_REQUIRE(order >= 0, "filter: order >= 0");
+    PF_REQUIRE(ncols >= 1, "filter: ncols >= 1");
+    PF_REQUIRE(ldx >= op->D && ldy >= op->D, "filter: leading dimension < D");
+    PF_REQUIRE(dtype == PF_F64 || dtype == PF_F32, "filter: dtype");
+    double phi0, phi1, a_tmp, b_tmp;
+    poly_phi(polytype, phi0, phi1);
+    poly_props(polytype, 1, a_tmp, b_tmp);
+    hipStream_t st = S(stream);
+    const Affine af = op->affine(true);
+    const size_t esz = dtype == PF_F64 ? 8 : 4;
+    const int cmax = chunk_cols(dtype);
+    for (int c0 = 0; c0 < ncols; c0 += cmax) {
+      const int s = std::min(cmax, ncols - c0);
+      const Layout lay = interleaved_layout(dtype, s);
+      const size_t blk = (size_t)op->D * s * esz;
+      char* base = reinterpret_cast<char*>(op->fbuf.get(4 * blk));
+      void* Xp = base;
+      void* b[3] = {base + blk, base + 2 * blk, base + 3 * blk};
+      pack_block(dtype, X + (int64_t)c0 * ldx, ldx, Xp, op->D, s, nullptr, st);
+      double* Yc = Y + (int64_t)c0 * ldy;
+      bool z2 = true, z3 = true;  // b2 / b3 known to be zero
+      for (int k = order; k >= 1; --k) {
+        double alpha, beta_k1, dummy;
+        poly_props(polytype, k, alpha, dummy);
+        poly_props(polytype, k + 1, dummy, beta_k1);
+        int flags = kHasX;
+        if (!z2) flags |= kHasH;
+        if (!z3 && beta_k1 != 0.0) flags |= kHasB3;
+        launch_step(op, dtype, flags, lay, s, af, coeffs[k], alpha, beta_k1, Xp, b[1], b[2], b[0],
+                    0, st);
+        void* t = b[0];
+        b[0] = b[2];
+        b[2] = b[1];
+        b[1] = t;
+        z3 = z2;
+        z2 = false;
+      }
+      double beta1, dummy;
+      poly_props(polytype, 1, dummy, beta1);
+      int flags = kHasX | kOutCM;
+      if (!z2) flags |= kHasH;
+      if (!z3 && beta1 * phi0 != 0.0) flags |= kHasB3;
+      launch_step(op, dtype, flags, lay, s, af, coeffs[0] * phi0, phi1, beta1 * phi0, Xp, b[1],
+                  b[2], Yc, ldy, st);
+    }
+  });
+}
+
+extern "C" int pf_fill_probes(int32_t kind, uint64_t seed, int64_t D, int32_t R, double* out,
+                              int64_t ld, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(out && D >= 1 && R >= 1 && R <= 65535 && ld >= D, "probes: bad shape");
+    PF_REQUIRE(kind == PF_PROBE_GAUSSIAN || kind == PF_PROBE_PHASE, "probes: bad kind");
+    hipLaunchKernelGGL(probes_kernel, dim3((unsigned)((D + 255) / 256), (unsigned)R), dim3(256), 0,
+                       S(stream), kind, seed, D, R, out, ld);
+    PF_HIP(hipGetLastError());
+  });
+}
+
+extern "C" int pf_kpm_moments(pf_op* op, int32_t dtype, int32_t N, int32_t R, const double* probes,
+                              int64_t ldp, double* mu_out, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(op && probes && mu_out, "kpm: null argument");
+    PF_REQUIRE(N >= 2 && R >= 1 && ldp >= op->D, "kpm: need N >= 2, R >= 1, ldp >= D");
+    PF_REQUIRE(dtype == PF_F64 || dtype == PF_F32, "kpm: dtype");
+    hipStream_t st = S(stream);
+    const Affine af = op->affine(true);
+    const size_t esz = dtype == PF_F64 ? 8 : 4;
+    const int cmax = chunk_cols(dtype);
+    const int dot_blocks = 1024;
+    // partials: [N][dot_blocks] for the moments + column norms + mu accumulator
+    const int nb_norm = 256;
+    double* part = reinterpret_cast<double*>(
+        op->kpart.get(((size_t)N * dot_blocks + (size_t)cmax * nb_norm + cmax + N) * sizeof(double)));
+    double* npart = part + (size_t)N * dot_blocks;
+    double* cscale = npart + (size_t)cmax * nb_norm;
+    double* mu_dev = cscale + cmax;
+    PF_HIP(hipMemsetAsync(mu_dev, 0, N * sizeof(double), st));
+    for (int c0 = 0; c0 < R; c0 += cmax) {
+      const int s = std::min(cmax, R - c0);
+      const Layout lay = interleaved_layout(dtype, s);
+      const size_t blk = (size_t)op->D * s * esz;
+      char* base = reinterpret_cast<char*>(op->fbuf.get(4 * blk));
+      void* P = base;  // normalized probes = T_0
+      void* t[3] = {base + blk, base + 2 * blk, base + 3 * blk};
+      const double* src = probes + (int64_t)c0 * ldp;
+      hipLaunchKernelGGL(colnorm2_kernel, dim3(nb_norm, s), dim3(256), 0, st, src, ldp, op->D, npart);
+      hipLaunchKernelGGL(inv_sqrt_sum_kernel, dim3(1), dim3(64), 0, st, npart, nb_norm, s, cscale);
+      PF_HIP(hipGetLastError());
+      pack_block(dtype, src, ldp, P, op->D, s, cscale, st);
+      // mu_0: <P, P>  (an H-free step that copies P and dots it)
+      launch_step(op, dtype, kHasX | kDot, lay, s, af, 1.0, 0.0, 0.0, P, nullptr, nullptr, t[0], 0,
+                  st, part + 0, P, dot_blocks);
+      // mu_1: T_1 = H~ T_0
+      launch_step(op, dtype, kHasH | kDot, lay, s, af, 0.0, 1.0, 0.0, nullptr, P, nullptr, t[1], 0,
+                  st, part + (size_t)1 * dot_blocks, P, dot_blocks);
+      // T_{n} = 2 H~ T_{n-1} - T_{n-2}; t[0] holds T_0 (copy), t[1] T_1
+      void* tm2 = t[0];
+      void* tm1 = t[1];
+      void* tn = t[2];
+      for (int n = 2; n < N; ++n) {
+        launch_step(op, dtype, kHasH | kHasB3 | kDot, lay, s, af, 0.0, 2.0, -1.0, nullptr, tm1,
+                    tm2, tn, 0, st, part + (size_t)n * dot_blocks, P, dot_blocks);
+        void* tmp = tm2;
+        tm2 = tm1;
+        tm1 = tn;
+        tn = tmp;
+      }
+      // every step wrote nblk block partials at stride dot_blocks
+      const int64_t ngroups = (op->D + lay.rows_per_block - 1) / lay.rows_per_block;
+      const int nblk = (int)std::min<int64_t>(ngroups, dot_blocks);
+      hipLaunchKernelGGL(kpm_reduce_kernel, dim3((N + 255) / 256), dim3(256), 0, st, part,
+                         dot_blocks, nblk, N, mu_dev);
+      PF_HIP(hipGetLastError());
+    }
+    op->khost.resize(N);
+    PF_HIP(hipMemcpyAsync(op->khost.data(), mu_dev, N * sizeof(double), hipMemcpyDeviceToHost, st));
+    PF_HIP(hipStreamSynchronize(st));
+    const double scale = (double)op->D / (double)R;
+    for (int n = 0; n < N; ++n) mu_out[n] = op->khost[n] * scale;
+  });
+}

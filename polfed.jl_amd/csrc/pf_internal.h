@@ -1,0 +1,179 @@
+// Internal declarations shared by the translation units of libpolfed_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/polfed_hip.h"
+#include "pf_rank.h"
+
+namespace pf {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string& msg);
+void clear_error();
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define PF_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw ::pf::Error{PF_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+#define PF_REQUIRE(cond, msg)                                         \
+  do {                                                                \
+    if (!(cond)) throw ::pf::Error{PF_ERR_INVALID, std::string(msg)}; \
+  } while (0)
+
+// Runs f, converting exceptions into a status code + thread-local message.
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    clear_error();
+    return PF_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return PF_ERR_NOMEM;
+  }
+}
+
+inline hipStream_t S(pf_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+// --------------------------------------------------- device scratch buffer
+// Grow-only device allocation owned by a handle or operator.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* get(size_t need) {
+    if (need > bytes) {
+      if (p) PF_HIP(hipFree(p));
+      p = nullptr;
+      bytes = 0;
+      if (hipMalloc(&p, need) != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        throw Error{PF_ERR_NOMEM, "hipMalloc of " + std::to_string(need) + " bytes failed"};
+      }
+      bytes = need;
+    }
+    return p;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// ----------------------------------------------------------- operator data
+constexpr int kMaxBonds = 128;
+constexpr int kMaxClasses = 8;
+constexpr int kMaxLoc = 27;
+constexpr int kMaxGrain = 4;
+
+// Spin-chain operator parameters, passed by value (kernel argument segment).
+struct ChainParams {
+  int32_t L, Lup, nb, ncls;
+  int64_t D;
+  int32_t cls_begin[kMaxClasses + 1];  // bonds grouped by off-diagonal value class
+  int8_t lo[kMaxBonds], hi[kMaxBonds];  // 0-based bit positions, lo < hi
+  double cval[kMaxClasses];             // unscaled off-diagonal value J/2 of each class
+};
+
+// Quantum-Sun operator parameters (spin 1/2, full space).
+struct QsunParams {
+  int32_t Lg, Lloc;
+  int64_t D;
+  uint32_t flip[kMaxLoc];  // XOR mask of outer spin j and its grain partner
+  double coup[kMaxLoc];    // unscaled off-diagonal value g0 * couplings[j] / 4
+  double h[kMaxLoc];       // z-fields of the outer spins
+  double G[(1 << kMaxGrain) * (1 << kMaxGrain)];  // grain matrix, G[a*dg + b]
+};
+
+enum OpKind { kChain = 0, kQsun = 1 };
+
+// Affine map of the operator: H' = (H - shift) * scale (scale = 1/a, shift = b
+// for the rescaled operator; 1, 0 for the plain one).
+struct Affine {
+  double scale, shift;
+};
+
+// Kernel flag bits of the fused step out = cx*X + ch*(H' b2) + cb*b3.
+enum : int {
+  kHasX = 1,    // read X
+  kHasH = 2,    // apply the operator to b2
+  kHasB3 = 4,   // read b3
+  kOutCM = 8,   // write the output column-major (FP64) instead of interleaved
+  kDot = 16,    // accumulate <P, out> per block (KPM)
+};
+
+struct Layout {
+  int64_t rs;      // row stride (s for interleaved, 1 for column-major)
+  int64_t cstride; // column stride for blockIdx.y (column-major), 0 for interleaved
+  int G;           // lanes per row
+  int rows_per_block;
+  int V;           // elements per lane
+};
+
+}  // namespace pf
+
+struct pf_handle {
+  int device = 0;
+  hipDeviceProp_t prop{};
+  pf::DevBuf partials;  // gram partial sums
+  pf::DevBuf small;     // small matrices staged to the device
+  pf::DevBuf vecs;      // residual / rotation scratch
+};
+
+struct pf_op {
+  pf_handle* h = nullptr;
+  pf::OpKind kind = pf::kChain;
+  int64_t D = 0;
+  double Emin = -1.0, Emax = 1.0;
+  pf::ChainParams chain{};
+  pf::QsunParams qsun{};
+  uint32_t* states = nullptr;  // spin chain: ascending basis states
+  uint32_t* binom = nullptr;   // spin chain: C(p, t), p < 32, t < kRankT
+  double* diag = nullptr;      // spin chain: unscaled diagonal, reference summation order
+  pf::DevBuf fbuf;             // filter / KPM row-interleaved work blocks
+  pf::DevBuf kpart;            // KPM partial dots
+  std::vector<double> khost;   // KPM host staging
+  pf::Affine affine(bool rescaled) const {
+    if (!rescaled) return {1.0, 0.0};
+    const double a = (Emax - Emin) / 2, b = (Emax + Emin) / 2;
+    return {1.0 / a, b};
+  }
+};
+
+namespace pf {
+
+// Launch one fused operator step over `ncols` columns (ops.hip).
+// out = cx*X + ch*(H' b2) + cb*b3 with H' = (H - shift)*scale.
+// T is the arithmetic type of X, b2, b3 (and of out unless kOutCM, which
+// writes FP64 column-major to `outp` with leading dimension out_ld).
+void launch_step(pf_op* op, int dtype, int flags, const Layout& lay, int ncols, Affine af,
+                 double cx, double ch, double cb, const void* X, const void* B2, const void* B3,
+                 void* out, int64_t out_ld, hipStream_t st, double* dot_part = nullptr,
+                 const void* P = nullptr, int dot_blocks = 0);
+
+Layout interleaved_layout(int dtype, int s);
+Layout colmajor_layout(int64_t ld);
+
+// Row-interleaved pack: dst[r*s + c] = (T)(src[r + c*ld] * colscale[c]) (colscale may be null).
+void pack_block(int dtype, const double* src, int64_t ld, void* dst, int64_t D, int s,
+                const double* colscale_dev, hipStream_t st);
+
+}  // namespace pf

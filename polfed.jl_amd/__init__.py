@@ -1,10 +1,19 @@
 """polfed_amd -- MI355X-native hot path of the POLFED eigensolver.
 
-Import name ``polfed_amd`` (the directory is ``polfed.jl_amd``; entry points
-register it with ``polfed_amd_loader``-style path setup).  The GPU work runs in
-``lib/libpolfed_hip.so`` (C ABI in ``include/polfed_hip.h``); this package is
-the host-side mirror of the reference's Julia API around it.
+Import name ``polfed_amd`` (the directory is ``polfed.jl_amd``; the test,
+bench and build entry points register it under that name).  The GPU work runs
+in ``lib/libpolfed_hip.so`` (C ABI in ``include/polfed_hip.h``); this package
+is the host-side mirror of the reference's Julia API around it.
 """
-from ._lib import PolfedHIPError, NumericBreakdown  # noqa: F401
+from ._lib import NumericBreakdown, PolfedHIPError  # noqa: F401
+from .configs import (DoSConfig, FactorizationConfig, FullRO, MappingConfig,  # noqa: F401
+                      MulColsParallel, NoParallel, PartialRO, PolfedDefaults, TransformConfig,
+                      TwoLevelParallel)
 
 __version__ = "0.1.0"
+
+
+def polfed(*args, **kwargs):
+    """See ``polfed_amd.core.polfed``."""
+    from .core import polfed as _polfed
+    return _polfed(*args, **kwargs)

@@ -10,17 +10,22 @@ from . import _lib
 from ._lib import check, dptr
 from .device import Device, colmajor
 
+MAX_N = 64  # widest block of one pf_gram / pf_gemm_update call
+
 
 def gram(dev: Device, A: torch.Tensor, B: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
     """G = A^T B (m x n) on the device; returned as an (n, m) tensor = column-major m x n."""
     pa, lda, D, m = colmajor(A)
-    pb, ldb, D2, n = colmajor(B)
+    _, _, D2, n = colmajor(B)
     if D != D2:
         raise ValueError("gram: row counts differ")
     if out is None:
         out = torch.empty((n, m), dtype=torch.float64, device=A.device)
-    pg, ldg, _, _ = colmajor(out)
-    check(dev.lib.pf_gram(dev.h, pa, lda, m, pb, ldb, n, D, pg, ldg, dev.stream))
+    for j0 in range(0, n, MAX_N):           # B columns are independent
+        Bj = B[j0:j0 + MAX_N] if B.dim() == 2 else B
+        pb, ldb, _, nj = colmajor(Bj)
+        pg, ldg, _, _ = colmajor(out[j0:j0 + nj])
+        check(dev.lib.pf_gram(dev.h, pa, lda, m, pb, ldb, nj, D, pg, ldg, dev.stream))
     return out
 
 
@@ -28,12 +33,17 @@ def update(dev: Device, A: torch.Tensor, G: torch.Tensor, B: torch.Tensor, alpha
            beta: float = 1.0) -> None:
     """B = beta B + alpha A G with G given as an (n, m) tensor (column-major m x n)."""
     pa, lda, D, m = colmajor(A)
-    pg, ldg, m2, n = colmajor(G)
-    pb, ldb, D2, n2 = colmajor(B)
+    _, _, m2, n = colmajor(G)
+    _, _, D2, n2 = colmajor(B)
     if m != m2 or n != n2 or D != D2:
         raise ValueError("update: shape mismatch")
-    check(dev.lib.pf_gemm_update(dev.h, pa, lda, m, pg, ldg, pb, ldb, n, D, float(alpha),
-                                 float(beta), dev.stream))
+    for j0 in range(0, n, MAX_N):           # output columns are independent
+        Gj = G[j0:j0 + MAX_N] if G.dim() == 2 else G
+        Bj = B[j0:j0 + MAX_N] if B.dim() == 2 else B
+        pg, ldg, _, nj = colmajor(Gj)
+        pb, ldb, _, _ = colmajor(Bj)
+        check(dev.lib.pf_gemm_update(dev.h, pa, lda, m, pg, ldg, pb, ldb, nj, D, float(alpha),
+                                     float(beta), dev.stream))
 
 
 def reorth(dev: Device, V: torch.Tensor, W: torch.Tensor, passes: int = 2) -> None:

@@ -1,0 +1,151 @@
+"""Configuration objects and defaults, mirroring the reference's keyword API.
+
+``PolfedDefaults`` (src/Polfed/Structs/PolfedDefaults.jl:61-217),
+``MappingConfig`` (src/Polfed/Structs/MappingConfig.jl:82-103),
+``TransformConfig`` (src/Polfed/Structs/TransformConfig.jl:77-100),
+``FactorizationConfig`` (src/Polfed/Structs/FactorizationConfig.jl:49-70),
+``DoSConfig`` (src/Polfed/Structs/DensitiesOfStatesConfig.jl:39-51), the
+parallelization tags (src/Polfed/Structs/Parallelization.jl) and the
+reorthogonalization tags (src/Lanczos/ReOrthTechnics/ReOrthTechnics.jl).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Callable, Optional
+
+
+# ------------------------------------------------------------------ defaults
+class PolfedDefaults:
+    """Default constants of the reference (PolfedDefaults.jl)."""
+
+    produce_report = False
+    optimize_mapping = False
+    polynomialtype = "Chebyshev"
+    cutoff = 0.17
+    normalization = 1.0
+    order_safety_factor = 0.97
+    overestimate_iters = 1.25
+    tol = 1e-14
+    eigentol = 1e-9
+    which = "LR"
+    kernel = "Jackson"
+    N = 250
+    R = 300
+
+    @staticmethod
+    def coefficients(lam: float, n: int) -> float:
+        """c_0 = 1, c_n = 2 cos(n acos(lam)) (PolfedDefaults.jl:104; the code,
+        not its docstring, which states the reverse)."""
+        return float((2 - (n == 0)) * math.cos(n * math.acos(lam)))
+
+    @staticmethod
+    def expectedkrylovdim(howmany: int, blocksize: int, eta: float) -> int:
+        """ceil((20.427 s + 1.696 howmany) eta) (PolfedDefaults.jl:61-62)."""
+        return math.ceil((20.427 * blocksize + 1.696 * howmany) * eta)
+
+
+# ---------------------------------------------------------- strategy tags
+class Parallelization:
+    """Tag types of the reference; on the GPU every call is ``NoParallel``
+    (src/Polfed/Plan/Plan.jl:64-67): one HIP launch covers the whole block."""
+
+
+class NoParallel(Parallelization):
+    pass
+
+
+class MulColsParallel(Parallelization):
+    def __init__(self, nt_per_col: int = 1):
+        self.nt_per_col = nt_per_col
+
+
+class TwoLevelParallel(Parallelization):
+    def __init__(self, nt_per_col: int = 1):
+        self.nt_per_col = nt_per_col
+
+
+class ReOrthTechnique:
+    pass
+
+
+class FullRO(ReOrthTechnique):
+    """Full reorthogonalization every iteration.  ``passes`` classical
+    Gram-Schmidt sweeps (the reference applies one; 2 = CGS2)."""
+
+    def __init__(self, passes: int = 2):
+        self.passes = passes
+
+
+class PartialRO(ReOrthTechnique):
+    """Periodic reorthogonalization (reorthogonalization.jl:33-48)."""
+
+    def __init__(self, skip: int, passes: int = 2):
+        self.skip = skip
+        self.passes = passes
+
+
+# ----------------------------------------------------------------- configs
+@dataclass
+class MappingConfig:
+    """Mapping stage options (MappingConfig.jl:14-61).
+
+    f_rescaled(Y, X), clenshaw_recurrence(b1, b2, b3, c, [k,] X) and
+    clenshaw_finalsum(b2, b3, c0, Y, X) are device-tensor callbacks with the
+    reference's signatures; Emin/Emax skip the extremal Lanczos probes.
+    """
+
+    parallel_strategy: Optional[Parallelization] = None
+    optimize_mapping: bool = PolfedDefaults.optimize_mapping
+    f_rescaled: Optional[Callable] = None
+    clenshaw_recurrence: Optional[Callable] = None
+    clenshaw_finalsum: Optional[Callable] = None
+    Emin: Optional[float] = None
+    Emax: Optional[float] = None
+
+
+@dataclass
+class TransformConfig:
+    """Polynomial filter options (TransformConfig.jl:6-100).  ``dtype``
+    selects the filter arithmetic on the GPU: "f64" (plain drop-in) or "f32"."""
+
+    coefficients: Callable = PolfedDefaults.coefficients
+    normalization: float = PolfedDefaults.normalization
+    cutoff: float = PolfedDefaults.cutoff
+    left: Optional[float] = None
+    right: Optional[float] = None
+    order: Optional[int] = None
+    order_safety_factor: float = PolfedDefaults.order_safety_factor
+    polynomialtype: str = PolfedDefaults.polynomialtype
+    dtype: str = "f64"
+
+
+@dataclass
+class FactorizationConfig:
+    """Krylov factorization options (FactorizationConfig.jl:49-70)."""
+
+    rot: ReOrthTechnique = None
+    which: str = PolfedDefaults.which
+    tol: float = PolfedDefaults.tol
+    eigentol: Optional[float] = PolfedDefaults.eigentol
+    overestimate_iters: float = PolfedDefaults.overestimate_iters
+
+    def __post_init__(self):
+        if self.rot is None:
+            self.rot = FullRO()
+
+
+@dataclass
+class DoSConfig:
+    """Density-of-states options (DensitiesOfStatesConfig.jl:39-51).
+
+    ``probe_kind`` "gaussian" (reference) or "phase"; ``seed`` drives the
+    counter-based probe generator; ``dtype`` the KPM arithmetic ("f64"/"f32").
+    """
+
+    N: int = PolfedDefaults.N
+    R: int = PolfedDefaults.R
+    kernel: str = PolfedDefaults.kernel
+    probe_kind: str = "gaussian"
+    seed: int = 0x5EED
+    dtype: str = "f64"

@@ -1,0 +1,232 @@
+"""``polfed(...)``: the public solver entry of the reference, on the GPU.
+
+Restates ``polfed(f!, x0, howmany, target; mapping, transform, fact, dos)``
+(src/Polfed/PolfedCore.jl:153-263): initial-state checks (:38-95), the mapping
+plan with Emin/Emax Lanczos probes (src/Polfed/Plan/Plan.jl:128-191), the
+transform plan (:203-236), ``polfed_algorithm`` (src/Polfed/Algorithm/polfed_algorithm.jl:15-37):
+KPM density of states -> interval and order -> block Lanczos on the filtered
+operator, with eigenvalues and residuals from the unscaled operator.
+
+The operator is a matrix-free ``Operator`` (``models``), so the filter runs as
+one fused HIP call per Lanczos iteration (``pf_filter``).  The
+``MappingConfig`` hooks (f_rescaled / clenshaw_recurrence / clenshaw_finalsum)
+replace the corresponding pieces with user callables, as in the reference
+(DefineTransformation.jl:92-168).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import blockops as bo
+from . import spectral as spc
+from .configs import (DoSConfig, FactorizationConfig, MappingConfig, PolfedDefaults,
+                      TransformConfig)
+from .device import Device, default_device
+from .lanczos import LanczosResult, lanczos
+from .operators import Operator
+
+
+@dataclass
+class Report:
+    """Spectral-transform, factorization and timing summary of a run
+    (the reference's Report, src/Polfed/Structs/Report/*.jl)."""
+    Emin: float
+    Emax: float
+    target: float
+    left: float
+    right: float
+    order: int
+    blocksize: int
+    maxdim: int
+    iterations: int
+    converged: int
+    eigenconverged: int
+    residual: float
+    eigenresidual: float
+    num_vecmuls: int
+    filter_dtype: str
+    moments: np.ndarray
+    times: dict = field(default_factory=dict)
+    lanczos: Optional[LanczosResult] = None
+
+
+def initial_state_tolerance(dtype=torch.float64) -> float:
+    """``_initial_state_tolerance`` (PolfedCore.jl:38-41)."""
+    eps = torch.finfo(dtype).eps
+    return max(1e-10 if dtype == torch.float64 else float(torch.tensor(1e-10, dtype=dtype)),
+               10 * eps)
+
+
+def prepare_initial_state(dev: Device, x0) -> torch.Tensor:
+    """``_prepare_polfed_initial_state`` (PolfedCore.jl:43-95), on the device.
+    Returns a (D,) vector or an (s, D) block; the caller's array is not mutated."""
+    if not isinstance(x0, torch.Tensor):
+        x0 = dev.to_device(np.asarray(x0, dtype=np.float64))
+    if x0.dim() == 1:
+        if x0.numel() == 0:
+            raise ValueError("POLFED initial vector must not be empty.")
+        nrm = float(np.sqrt(bo.coldots(dev, x0, x0)[0]))
+        if not np.isfinite(nrm):
+            raise ValueError("POLFED initial vector must contain finite values.")
+        if nrm == 0:
+            raise ValueError("POLFED initial vector must be nonzero.")
+        tol = initial_state_tolerance()
+        if abs(nrm - 1.0) <= max(tol, tol * max(nrm, 1.0)):
+            return x0
+        return x0 * (1.0 / nrm)
+    s, D = x0.shape
+    if D == 0:
+        raise ValueError("POLFED initial matrix must have at least one row.")
+    if s == 0:
+        raise ValueError("POLFED initial matrix must have at least one column.")
+    if s > D:
+        raise ValueError("POLFED initial matrix cannot have orthonormal columns when it has "
+                         "more columns than rows.")
+    G = bo.gram(dev, x0, x0).cpu().numpy().T
+    if not np.all(np.isfinite(G)):
+        raise ValueError("POLFED initial matrix must contain finite values.")
+    tol = initial_state_tolerance()
+    I = np.eye(s)
+    if np.linalg.norm(G - I) <= max(tol, tol * max(np.linalg.norm(G), np.linalg.norm(I))):
+        return x0
+    q = x0.clone()
+    try:
+        bo.block_qr(dev, q)
+    except Exception:
+        Q, _ = np.linalg.qr(x0.cpu().numpy().T)
+        q = torch.from_numpy(np.ascontiguousarray(Q.T)).to(x0.device)
+    return q
+
+
+def _extremal(dev: Device, op: Operator, v0: torch.Tensor, which: str) -> float:
+    """``lanczos(f!, v0, 1; which, maxdim=1000)`` probe of build_mapping_plan
+    (Plan.jl:136-168) with the unscaled operator."""
+    f = lambda X, Y: op.apply(X, Y)
+    res = lanczos(dev, f, v0, 1, maxdim=1000, which=which, tol=1e-14, eigentol=1e-8,
+                  mapvals=op, stage_timers=False)
+    if res.vals.size == 0:
+        raise RuntimeError(f"Failed to estimate {which} extremal eigenvalue with Lanczos. "
+                           "Set MappingConfig(Emin=..., Emax=...) explicitly.")
+    return float(res.vals[0] if which == "SR" else res.vals[-1])
+
+
+def _filter_callable(op: Operator, mapping: MappingConfig, coeffs: np.ndarray, polytype: str,
+                     dtype: str):
+    """The transformed operator f!_transformed (DefineTransformation.jl:11-48)."""
+    if mapping.clenshaw_recurrence is None and mapping.clenshaw_finalsum is None \
+            and mapping.f_rescaled is None:
+        return lambda X, Y: op.filter(coeffs, X, Y, dtype=dtype, polytype=polytype)
+    # user hooks: Clenshaw loop on the host, each piece on the device
+    alpha, beta, phi0, phi1 = spc._poly_alpha_beta(polytype)
+    K = len(coeffs) - 1
+
+    def rec(b1, b2, b3, c, k, X):
+        if mapping.clenshaw_recurrence is not None:
+            fn = mapping.clenshaw_recurrence
+            try:
+                fn(b1, b2, b3, c, k, X)
+            except TypeError:
+                fn(b1, b2, b3, c, X)
+        elif mapping.f_rescaled is not None:
+            mapping.f_rescaled(b1, b2)
+            b1.mul_(alpha(k)).add_(X, alpha=c).add_(b3, alpha=beta(k + 1))
+        else:
+            op.recurrence(b1, b2, b3, c, X, alpha=alpha(k), beta=beta(k + 1))
+
+    def fin(b2, b3, c0, Y, X):
+        if mapping.clenshaw_finalsum is not None:
+            mapping.clenshaw_finalsum(b2, b3, c0, Y, X)
+        elif mapping.f_rescaled is not None:
+            mapping.f_rescaled(Y, b2)
+            Y.mul_(phi1).add_(X, alpha=c0 * phi0).add_(b3, alpha=beta(1) * phi0)
+        else:
+            op.finalsum(b2, b3, c0, Y, X, beta1=beta(1), phi0=phi0, phi1=phi1)
+
+    def apply(X, Y):
+        b = [torch.zeros_like(X) for _ in range(3)]
+        for k in range(K, 0, -1):
+            rec(b[0], b[1], b[2], float(coeffs[k]), k, X)
+            b = [b[2], b[0], b[1]]
+        fin(b[1], b[2], float(coeffs[0]), Y, X)
+
+    return apply
+
+
+def polfed(op: Operator, x0, howmany: int, target, *, produce_report: bool = False,
+           mapping: Optional[MappingConfig] = None, transform: Optional[TransformConfig] = None,
+           fact: Optional[FactorizationConfig] = None, dos: Optional[DoSConfig] = None,
+           probes: Optional[torch.Tensor] = None, device: Optional[Device] = None):
+    """Eigenpairs of ``op`` closest (by the polynomial filter) to ``target``.
+
+    Returns ``(vals, vecs)`` -- vals ascending (host numpy), vecs an
+    (nconv, D) device block -- or ``(vals, vecs, report)`` with
+    ``produce_report=True``.  ``probes`` (D x R device block) overrides the
+    generated KPM probe vectors.
+    """
+    mapping = mapping or MappingConfig()
+    transform = transform or TransformConfig()
+    fact = fact or FactorizationConfig()
+    dos = dos or DoSConfig()
+    dev = device or op.device or default_device()
+    t_all = time.perf_counter()
+    times = {}
+
+    x0 = prepare_initial_state(dev, x0)
+    s = 1 if x0.dim() == 1 else x0.shape[0]
+    D = x0.shape[-1]
+    if D != op.D:
+        raise ValueError(f"x0 has {D} rows, operator dimension is {op.D}")
+
+    # ---- mapping plan (Plan.jl:128-191)
+    t0 = time.perf_counter()
+    v0 = x0 if x0.dim() == 1 else x0[0]
+    Emin = mapping.Emin if mapping.Emin is not None else _extremal(dev, op, v0, "SR")
+    Emax = mapping.Emax if mapping.Emax is not None else _extremal(dev, op, v0, "LR")
+    op.set_rescale(Emin, Emax)
+    a, b = (Emax - Emin) / 2, (Emax + Emin) / 2
+    torch.cuda.synchronize()
+    times["bounds"] = time.perf_counter() - t0
+
+    plan = spc.build_transform_plan(transform, a, b, howmany, target)
+    maxdim = PolfedDefaults.expectedkrylovdim(howmany, s, fact.overestimate_iters)
+
+    # ---- getdos! (DensitiesOfStates.jl:14-43): KPM moments on the device
+    t0 = time.perf_counter()
+    if probes is None:
+        probes = bo.fill_probes(dev, D, dos.R, dos.seed, dos.probe_kind)
+    moments = op.kpm_moments(dos.N, probes, dtype=dos.dtype)
+    rho = spc.DensityOfStates(moments, dos.kernel)
+    times["dos"] = time.perf_counter() - t0
+
+    # ---- getspectraltransform! (SpectralTransformation.jl:19-31)
+    t0 = time.perf_counter()
+    spc.getbounds(plan, a, b, rho, "mean")
+    spc.getorderofexpansion(plan)
+    coeffs = spc.normalized_coefficients(plan.coefficients, plan.target, plan.order,
+                                         plan.normalization, plan.polynomialtype)
+    times["transform"] = time.perf_counter() - t0
+    f_t = _filter_callable(op, mapping, coeffs, plan.polynomialtype, transform.dtype)
+
+    # ---- lanczos on the filtered operator (polfed_algorithm.jl:25-34)
+    t0 = time.perf_counter()
+    res = lanczos(dev, f_t, x0, howmany, maxdim=maxdim, which=fact.which, tol=fact.tol,
+                  eigentol=fact.eigentol if fact.eigentol is not None else 1e-8, mapvals=op,
+                  rot=fact.rot)
+    torch.cuda.synchronize()
+    times["lanczos"] = time.perf_counter() - t0
+    times.update({f"lanczos.{k}": v for k, v in res.times.items()})
+    times["total"] = time.perf_counter() - t_all
+
+    if not produce_report:
+        return res.vals, res.vecs
+    iters_needed = res.numiter
+    report = Report(Emin, Emax, plan.target, plan.left, plan.right, plan.order, s, maxdim,
+                    res.numiter, res.converged, res.eigenconverged, res.residual,
+                    res.eigenresidual, plan.order * s * iters_needed + howmany,
+                    transform.dtype, moments, times, res)
+    return res.vals, res.vecs, report

@@ -1,0 +1,247 @@
+"""Block Lanczos on the GPU (FP64), mirroring the reference's factorization.
+
+Restates ``lanczos`` (src/Lanczos/Lanczos.jl:119-143), ``lanczos_algorithm!``
+(LanczosAlgorithm.jl:11-54), the step sequence of ``lanczositer!``
+(Factorization/Factorization.jl:75-96), the block operations of
+BlockLanczosFactorization.jl / FactorizationSteps/*.jl, the convergence checks
+of Convergence/ConvergenceInfo.jl, CalculatingResiduals.jl, NextChecking.jl,
+getvecs.jl, getvals.jl and the final ordering of LanczosMethod.jl:55-83.
+
+Device work (all FP64): basis storage (one D x maxdim column-major tensor),
+QR of the residual block (Cholesky-QR2, Householder on the host only when the
+block is numerically rank deficient), overlaps and three-term updates on the
+FP64 matrix cores, full reorthogonalization (``passes`` CGS sweeps), Ritz
+vectors and residuals.  Host work: the projected k x k eigenproblem.
+A vector x0 runs the same code with block size 1 (the reference's scalar
+factorization normalizes by ||r|| > 0, which is what Cholesky-QR returns).
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.linalg as sla
+import torch
+
+from . import blockops as bo
+from ._lib import NumericBreakdown
+from .configs import FullRO, PartialRO
+from .device import Device
+
+
+@dataclass
+class LanczosResult:
+    vals: np.ndarray
+    vecs: torch.Tensor           # (nconv, D) block = column-major D x nconv
+    residuals: np.ndarray        # ||H psi - E psi|| of the returned pairs
+    converged: int
+    eigenconverged: int
+    numiter: int
+    maxiter: int
+    krylovdim: int
+    residual: float
+    eigenresidual: float
+    converged_history: list = field(default_factory=list)
+    krylovdim_history: list = field(default_factory=list)
+    maxresidual_history: list = field(default_factory=list)
+    times: dict = field(default_factory=dict)
+
+
+def sortvals(vals: np.ndarray, which: str) -> np.ndarray:
+    """``sortvals`` (Convergence/EigSorter.jl:41-57)."""
+    if which == "SR":
+        return np.argsort(vals, kind="stable")
+    if which == "LR":
+        return np.argsort(-vals, kind="stable")
+    if which == "LM":
+        return np.argsort(-np.abs(vals), kind="stable")
+    if which == "SM":
+        return np.argsort(np.abs(vals), kind="stable")
+    raise ValueError(f"Unknown sorting mode: {which}")
+
+
+def _next_check(k: int, residual: float) -> int:
+    """``next_checking!`` (Convergence/NextChecking.jl:23-42)."""
+    if residual > 1e-2:
+        return k + 20
+    if residual > 1e-3:
+        return k + 10
+    return k + 5
+
+
+def _to_dev_matrix(M: np.ndarray, like: torch.Tensor) -> torch.Tensor:
+    """host m x n -> device (n, m) tensor (column-major m x n)."""
+    return torch.from_numpy(np.ascontiguousarray(M.T)).to(like.device)
+
+
+def _host_qr(block: torch.Tensor) -> np.ndarray:
+    """Householder QR on the host for a rank-deficient block (in place on the
+    device tensor); the reference's qr(fact.r) (BlockLanczosFactorization.jl:123)."""
+    A = block.detach().cpu().numpy().T
+    Q, R = np.linalg.qr(A)
+    block.copy_(torch.from_numpy(np.ascontiguousarray(Q.T)).to(block.device))
+    return R
+
+
+class _Timer:
+    def __init__(self, sync: bool):
+        self.sync = sync
+        self.t = {}
+
+    def __call__(self, key):
+        timer = self
+
+        class _Ctx:
+            def __enter__(self):
+                if timer.sync:
+                    torch.cuda.synchronize()
+                self.t0 = time.perf_counter()
+
+            def __exit__(self, *a):
+                if timer.sync:
+                    torch.cuda.synchronize()
+                timer.t[key] = timer.t.get(key, 0.0) + time.perf_counter() - self.t0
+
+        return _Ctx()
+
+
+def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | None = None,
+            which: str = "SR", tol: float = 1e-14, eigentol: float = 1e-8, mapvals=None,
+            rot=None, stage_timers: bool = True) -> LanczosResult:
+    """Block Lanczos for ``howmany`` pairs of the operator ``f``.
+
+    ``f(X, Y)`` writes the operator applied to the (s, D) block X into Y.
+    ``mapvals`` (default ``f``'s operator) evaluates the final Rayleigh
+    quotients and residuals: an ``Operator`` (unscaled H) or a callable
+    ``mapvals(X, Y)``.
+    """
+    rot = rot if rot is not None else FullRO()
+    if maxdim is None:
+        maxdim = 500 * howmany
+    X0 = x0 if x0.dim() == 2 else x0.reshape(1, -1)
+    s, D = X0.shape
+    maxiter = math.ceil(maxdim / s)
+    maxdim = maxiter * s
+    tm = _Timer(stage_timers)
+    basis = torch.empty((maxdim, D), dtype=torch.float64, device=X0.device)
+    mat = np.zeros((maxdim, maxdim))
+    r = torch.empty((s, D), dtype=torch.float64, device=X0.device)
+
+    def sym_upper(a):
+        return np.triu(a) + np.triu(a, 1).T
+
+    def overlap(V, W):
+        return bo.gram(dev, V, W).cpu().numpy().T
+
+    # ---- lanczositer (Factorization.jl:50-67)
+    basis[0:s].copy_(X0)
+    k = s
+    with tm("mapping"):
+        f(basis[0:s], r)
+    a = sym_upper(overlap(basis[0:s], r))
+    mat[0:s, 0:s] = a
+    bo.update(dev, basis[0:s], _to_dev_matrix(a, r), r, alpha=-1.0, beta=1.0)
+    numiter = 1
+    nextcheck = max(howmany // s + 1, 2)
+    converged = eigenconverged = 0
+    residual = eigenresidual = 1.0
+    hist_c, hist_k, hist_r = [], [], []
+    vals = np.zeros(0)
+    vecs = torch.empty((0, D), dtype=torch.float64, device=X0.device)
+    res_out = np.zeros(0)
+
+    while True:
+        # [1] tridiagonalization!: QR of r into the next basis slot
+        with tm("tridiagonalization"):
+            slot = basis[k:k + s]
+            slot.copy_(r)
+            try:
+                R = bo.block_qr(dev, slot)
+            except NumericBreakdown:
+                R = _host_qr(slot)
+            mat[k:k + s, k - s:k] = R
+            mat[k - s:k, k:k + s] = R.T
+            k += s
+        V_last, V_prev = basis[k - s:k], basis[k - 2 * s:k - s]
+        # [2] mapping!
+        with tm("mapping"):
+            f(V_last, r)
+        # [3] calculate_overlap!  [4] orthogonalization!
+        with tm("orthogonalization"):
+            a = sym_upper(overlap(V_last, r))
+            mat[k - s:k, k - s:k] = a
+            b = mat[k - s:k, k - 2 * s:k - s]
+            bo.update(dev, V_last, _to_dev_matrix(a, r), r, alpha=-1.0, beta=1.0)
+            bo.update(dev, V_prev, _to_dev_matrix(b.T, r), r, alpha=-1.0, beta=1.0)
+        # [5] reorthogonalization! against all but the last two blocks
+        with tm("reorthogonalization"):
+            nb = k - 2 * s
+            do_ro = nb > 0
+            if isinstance(rot, PartialRO):
+                it = k // s
+                do_ro = do_ro and (it % (2 + rot.skip) == 0 or (it - 1) % (2 + rot.skip) == 0)
+            if do_ro:
+                bo.reorth(dev, basis[0:nb], r, passes=rot.passes)
+        # [6] getnumberofconvergedvecs!
+        numiter += 1
+        if nextcheck == numiter or numiter >= maxiter:
+            with tm("convergence"):
+                T = mat[:k, :k]
+                lam, phi = sla.eigh(T, lower=False, check_finite=False)
+                idxs = sortvals(lam, which)
+                bnorm = mat[k - s:k, k - 2 * s:k - s]
+                converged = 0
+                maxres = 0.0
+                for i in range(howmany):
+                    rr = float(np.linalg.norm(bnorm @ phi[k - s:k, idxs[i]]))
+                    if rr > maxres:
+                        maxres = rr
+                    if maxres < tol:
+                        converged += 1
+                residual = maxres
+                hist_c.append(converged)
+                hist_k.append(k)
+                hist_r.append(maxres)
+                if converged == howmany or numiter >= maxiter:
+                    Phi = phi[:, idxs[:converged]]
+                    vecs = torch.empty((converged, D), dtype=torch.float64, device=X0.device)
+                    if converged:
+                        bo.update(dev, basis[0:k], _to_dev_matrix(Phi, r), vecs, alpha=1.0,
+                                  beta=0.0)
+                        vals, res_out = _eigen_residuals(dev, mapvals, vecs)
+                    else:
+                        vals, res_out = np.zeros(0), np.zeros(0)
+                    eigenconverged = 0
+                    for rv in res_out:
+                        if rv < eigentol:
+                            eigenconverged += 1
+                        else:
+                            break
+                    eigenresidual = float(res_out.max()) if res_out.size else 0.0
+                nextcheck = _next_check(numiter, residual)
+        if converged == howmany or numiter >= maxiter:
+            break
+
+    nconv = min(converged, eigenconverged)
+    order = np.argsort(vals[:nconv], kind="stable")
+    idx = torch.as_tensor(order, device=X0.device, dtype=torch.long)
+    vecs_sorted = vecs[:nconv].index_select(0, idx) if nconv else vecs[:0]
+    return LanczosResult(vals[:nconv][order], vecs_sorted, res_out[:nconv][order], converged,
+                         eigenconverged, numiter, maxiter, k, residual, eigenresidual, hist_c,
+                         hist_k, hist_r, dict(tm.t))
+
+
+def _eigen_residuals(dev: Device, mapvals, vecs: torch.Tensor):
+    """Rayleigh quotients and residual norms with the original operator
+    (getvals.jl:10-36, ConvergenceInfo.jl:139-173)."""
+    from .operators import Operator
+    if isinstance(mapvals, Operator):
+        return bo.residuals(dev, mapvals, vecs)
+    Hv = torch.empty_like(vecs)
+    mapvals(vecs, Hv)
+    E = bo.coldots(dev, vecs, Hv)
+    R = Hv - torch.from_numpy(E).to(vecs.device)[:, None] * vecs
+    res = np.sqrt(bo.coldots(dev, R, R))
+    return E, res

@@ -1,0 +1,115 @@
+"""End-to-end POLFED on the GPU (FP64 drop-in path) against the CPU oracle and
+exact spectra: same eigenvalue set, <= 1e-10 x spectral width, residuals <= 1e-10,
+Ritz subspace angles <= 1e-8 per degenerate cluster."""
+import numpy as np
+import pytest
+
+from oracle import clenshaw as oc
+from oracle import models as om
+from oracle import polfed as opf
+
+pytestmark = pytest.mark.gpu
+
+
+def _expected_set(ev, Emin, Emax, coeffs, nev):
+    a, b = (Emax - Emin) / 2, (Emax + Emin) / 2
+    p = oc.chebyshev_filter_dense((ev - b) / a, coeffs)
+    return np.sort(ev[np.argsort(-p, kind="stable")[:nev]])
+
+
+def _max_angle(V1, V2):
+    """largest principal angle between the column spans, from its sine
+    ||(I - Q2 Q2^T) Q1||_2 (arccos of the cosines loses everything below ~1e-8)."""
+    Q1, _ = np.linalg.qr(V1)
+    Q2, _ = np.linalg.qr(V2)
+    return float(np.arcsin(min(1.0, np.linalg.norm(Q1 - Q2 @ (Q2.T @ Q1), 2))))
+
+
+def test_polfed_block_fixed_order_matches_exact(dev):
+    from polfed_amd import DoSConfig, FactorizationConfig, MappingConfig, TransformConfig, polfed
+    from polfed_amd.models import xxz_hamiltonian
+    L, nev, s = 12, 20, 4
+    rng = np.random.default_rng(3)
+    h = om.resolve_fields(L, 0.0, 2.0, rng=rng)
+    op = xxz_hamiltonian(L, L // 2, 1.0, 1.0, 2.0, fields=h, device=dev)
+    H = om.xxz_hamiltonian(L, L // 2, 1.0, 1.0, 2.0, fields=h)
+    ev, V = np.linalg.eigh(H.toarray())
+    q, _ = np.linalg.qr(np.random.default_rng(4).standard_normal((op.D, s)))
+    vals, vecs, rep = polfed(op, q, nev, ("rescaled", 0.1), produce_report=True,
+                             mapping=MappingConfig(Emin=ev[0], Emax=ev[-1]),
+                             transform=TransformConfig(order=200),
+                             fact=FactorizationConfig(eigentol=1e-10), dos=DoSConfig(N=64, R=20))
+    assert rep.order == 200
+    assert len(vals) == nev
+    width = ev[-1] - ev[0]
+    exp = _expected_set(ev, ev[0], ev[-1], oc.filter_coefficients(0.1, 200), nev)
+    assert np.max(np.abs(np.sort(vals) - exp)) <= 1e-10 * width
+    psi = dev.to_host(vecs)
+    res = np.linalg.norm(H @ psi - psi * vals, axis=0)
+    assert res.max() <= 1e-10
+    # subspace angle against the exact eigenvectors of the same eigenvalues
+    idx = [int(np.argmin(np.abs(ev - v))) for v in vals]
+    assert _max_angle(psi, V[:, idx]) <= 1e-8
+    # the oracle (reference restatement) on the same problem returns the same set
+    ref = opf.polfed(lambda X: H @ X, q, nev, ("rescaled", 0.1), Emin=ev[0], Emax=ev[-1], order=200,
+                     eigentol=1e-10, N=64, R=20, rng=np.random.default_rng(5))
+    assert len(ref.vals) == nev
+    assert np.max(np.abs(np.sort(vals) - ref.vals)) <= 1e-10 * width
+
+
+def test_polfed_vector_defaults_matches_oracle(dev):
+    """Config-1 shape (L=14, D=3432, single-vector Lanczos, nev=50, middle
+    target, default interval/order selection) with the same KPM probes."""
+    from polfed_amd import FactorizationConfig, polfed
+    from polfed_amd.blockops import fill_probes
+    from polfed_amd.models import xxz_hamiltonian
+    L = 14
+    h = om.resolve_fields(L, 0.0, 5.0, rng=np.random.default_rng(0))
+    op = xxz_hamiltonian(L, L // 2, 1.0, 1.0, 5.0, fields=h, device=dev)
+    H = om.xxz_hamiltonian(L, L // 2, 1.0, 1.0, 5.0, fields=h)
+    x0 = np.random.default_rng(100).standard_normal(op.D)
+    x0 /= np.linalg.norm(x0)
+    P = fill_probes(dev, op.D, 300, seed=77)
+    vals, vecs, rep = polfed(op, x0, 50, ("rescaled", 0.0), produce_report=True,
+                             fact=FactorizationConfig(eigentol=1e-10), probes=P)
+    ref = opf.polfed(lambda X: H @ X, x0, 50, ("rescaled", 0.0), eigentol=1e-10,
+                     probes=dev.to_host(P))
+    ev = np.linalg.eigvalsh(H.toarray())
+    width = ev[-1] - ev[0]
+    assert abs(rep.Emin - ev[0]) < 1e-9 * width and abs(rep.Emax - ev[-1]) < 1e-9 * width
+    assert abs(rep.order - ref.plan.order) <= 1
+    assert len(vals) == 50 and len(ref.vals) == 50
+    exp = _expected_set(ev, rep.Emin, rep.Emax, oc.filter_coefficients(rep.target, rep.order), 50)
+    assert np.max(np.abs(np.sort(vals) - exp)) <= 1e-10 * width
+    if rep.order == ref.plan.order:
+        assert np.max(np.abs(vals - ref.vals)) <= 1e-10 * width
+    psi = dev.to_host(vecs)
+    assert np.linalg.norm(H @ psi - psi * vals, axis=0).max() <= 1e-10
+
+
+def test_polfed_qsun_block(dev):
+    """Quantum Sun, block size 2: the reference's own algorithm exhausts its
+    iterations on this problem and returns 13 of 16 pairs; the GPU path must
+    return the same pairs."""
+    from polfed_amd import FactorizationConfig, polfed
+    from polfed_amd.blockops import fill_probes
+    from polfed_amd.models import qsun_hamiltonian
+    draws = om.qsun_draws(7, 3, 0.75, rng=np.random.default_rng(8))
+    op = qsun_hamiltonian(7, 3, 1.0, 0.75, draws=draws, device=dev)
+    H = om.qsun_hamiltonian_from_draws(7, 3, 1.0, *draws)
+    ev = np.linalg.eigvalsh(H.toarray())
+    q, _ = np.linalg.qr(np.random.default_rng(1).standard_normal((op.D, 2)))
+    P = fill_probes(dev, op.D, 300, seed=91)
+    vals, vecs, rep = polfed(op, q, 16, "middle", produce_report=True,
+                             fact=FactorizationConfig(eigentol=1e-10), probes=P)
+    ref = opf.polfed(lambda X: H @ X, q, 16, "middle", eigentol=1e-10, probes=dev.to_host(P))
+    width = ev[-1] - ev[0]
+    assert abs(rep.order - ref.plan.order) <= 1
+    assert len(vals) == len(ref.vals) >= 10
+    if rep.order == ref.plan.order:
+        assert np.max(np.abs(vals - ref.vals)) <= 1e-10 * width
+    exp = _expected_set(ev, rep.Emin, rep.Emax, oc.filter_coefficients(rep.target, rep.order),
+                        len(vals))
+    assert np.max(np.abs(np.sort(vals) - exp)) <= 1e-10 * width
+    psi = dev.to_host(vecs)
+    assert np.linalg.norm(H @ psi - psi * vals, axis=0).max() <= 1e-10

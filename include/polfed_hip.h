@@ -147,6 +147,25 @@ int pf_filter(pf_op* op, int32_t dtype, int32_t polytype, const double* coeffs, 
               const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t ncols,
               pf_stream stream);
 
+/* ------------------------------------------------------------- tuning */
+
+enum { PF_TUNE_XCD_REMAP = 1, PF_TUNE_VEC = 2, PF_TUNE_RESET = 3 };
+
+/* Force a launch knob of the row-interleaved step kernels for this operator:
+ * PF_TUNE_XCD_REMAP (-1 default, 0 off, 1 on: contiguous row ranges per XCD),
+ * PF_TUNE_VEC (0 default, or 1/2/4/8 elements per lane), PF_TUNE_RESET (drop
+ * forced and autotuned settings).  No reference counterpart: launch tuning
+ * of this library. */
+int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value);
+
+/* Autotuner: times `nsteps` fused recurrence steps for each candidate launch
+ * configuration on this operator's shape (dtype, ncols columns) and keeps the
+ * fastest for later pf_filter / pf_kpm_moments calls.  Outputs (host, may be
+ * NULL): best time per step in microseconds, chosen vector width and XCD
+ * mapping.  No reference counterpart. */
+int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nsteps, double* best_us,
+                int32_t* best_vec, int32_t* best_xcd, pf_stream stream);
+
 /* ------------------------------------------------------------------- KPM */
 
 /* Fill R probe columns (device, column-major, ld) with a counter-based

@@ -26,6 +26,10 @@ PF_LEGENDRE = 1
 PF_HERMITE = 2
 PF_TAYLOR = 3
 
+PF_TUNE_XCD_REMAP = 1
+PF_TUNE_VEC = 2
+PF_TUNE_RESET = 3
+
 PF_PROBE_GAUSSIAN = 0
 PF_PROBE_PHASE = 1
 
@@ -74,6 +78,8 @@ SIGNATURES = {
     "pf_clenshaw_finalsum": (_c.c_int, [_vp, _i32, _vp, _vp, _dbl, _dbl, _dbl, _dbl, _vp, _vp,
                                         _i64, _i32, _vp]),
     "pf_filter": (_c.c_int, [_vp, _i32, _i32, _pdbl, _i32, _vp, _i64, _vp, _i64, _i32, _vp]),
+    "pf_op_set_tuning": (_c.c_int, [_vp, _i32, _i32]),
+    "pf_autotune": (_c.c_int, [_vp, _i32, _i32, _i32, _pdbl, _pi32, _pi32, _vp]),
     "pf_fill_probes": (_c.c_int, [_i32, _u64, _i64, _i32, _vp, _i64, _vp]),
     "pf_kpm_moments": (_c.c_int, [_vp, _i32, _i32, _i32, _vp, _i64, _pdbl, _vp]),
     "pf_gram": (_c.c_int, [_vp, _vp, _i64, _i32, _vp, _i64, _i32, _i64, _vp, _i32, _vp]),

@@ -27,10 +27,18 @@ template <typename T, int V> struct VT;
 template <> struct VT<double, 1> { typedef double t; };
 template <> struct VT<double, 2> { typedef double t __attribute__((ext_vector_type(2))); };
 template <> struct VT<double, 4> { typedef double t __attribute__((ext_vector_type(4))); };
+template <> struct VT<double, 8> { typedef double t __attribute__((ext_vector_type(8))); };
+template <> struct VT<double, 16> { typedef double t __attribute__((ext_vector_type(16))); };
 template <> struct VT<float, 1> { typedef float t; };
 template <> struct VT<float, 2> { typedef float t __attribute__((ext_vector_type(2))); };
 template <> struct VT<float, 4> { typedef float t __attribute__((ext_vector_type(4))); };
 template <> struct VT<float, 8> { typedef float t __attribute__((ext_vector_type(8))); };
+template <> struct VT<float, 16> { typedef float t __attribute__((ext_vector_type(16))); };
+template <> struct VT<float, 32> { typedef float t __attribute__((ext_vector_type(32))); };
+
+// Neighbour gathers issued together before they are accumulated, so their
+// memory latencies overlap instead of serializing on one wait per bond.
+constexpr int kGatherBatch = 4;  // Quantum Sun outer-spin flips per batch
 
 template <typename T, int V>
 __device__ __forceinline__ typename VT<T, V>::t vload(const T* p) {
@@ -47,6 +55,14 @@ __device__ __forceinline__ T vget(typename VT<T, V>::t v, int i) {
   } else {
     return v[i];
   }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; this bijection gives
+// XCD x a contiguous range of row groups, so the neighbour gathers of nearby
+// rows hit that XCD's own L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t n) {
+  const int64_t x = b & 7, i = b >> 3, q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + i;
 }
 
 // Block-wide sum of one double per thread (256 threads), result in thread 0.
@@ -90,8 +106,80 @@ __device__ __forceinline__ void step_epilogue(int flags, int64_t row, int64_t rs
 }
 
 // ------------------------------------------------------ spin-chain kernel
+// Broadcast a 32-bit value from lane Q of this lane's group of G lanes (the
+// G lanes that share one row).  G <= 4 uses a DPP quad permutation, G = 8 a
+// cross-lane permute.
+template <int G, int Q>
+__device__ __forceinline__ uint32_t group_bcast(uint32_t v) {
+  if constexpr (G <= 1) {
+    return v;
+  } else if constexpr (G == 2) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, Q | (Q << 2) | ((Q + 2) << 4) | ((Q + 2) << 6),
+                                              0xF, 0xF, false);
+  } else if constexpr (G == 4) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, Q * 0x55, 0xF, 0xF, false);
+  } else {
+    const int lane = threadIdx.x & 63;
+    return (uint32_t)__shfl((int)v, (lane & ~(G - 1)) + Q, 64);
+  }
+}
+
+// Ranks of the neighbours of bonds b0 + j, j < NBAT, as seen by every lane of
+// the row group: with GM in {1,2,4,8} lane g computes the bonds j = g + GM*u
+// and the group shares them; GM = 0 (any other group size) computes all NBAT.
+// A bond whose two spins are parallel yields the row's own rank r.
+template <int GM, int NBAT>
+__device__ __forceinline__ void bond_ranks(const ChainParams& P, int b0, uint32_t st, uint32_t r,
+                                           const uint32_t* C, int g, uint32_t (&out)[NBAT]) {
+  if constexpr (GM == 0) {
+#pragma unroll
+    for (int j = 0; j < NBAT; ++j) {
+      const int b = b0 + j;
+      uint32_t rn = r;
+      if (b < P.nb) {  // uniform
+        const int lo = P.lo[b], hi = P.hi[b];
+        if (((st >> lo) ^ (st >> hi)) & 1u) rn = neighbor_rank(st, r, lo, hi, C);
+      }
+      out[j] = rn;
+    }
+  } else {
+    constexpr int UB = NBAT / GM;
+    uint32_t mine[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const int b = b0 + g + GM * u;
+      uint32_t rn = r;
+      if (b < P.nb) {
+        const int lo = P.lo[b], hi = P.hi[b];  // per-lane bond: scalar table lookup
+        if (((st >> lo) ^ (st >> hi)) & 1u) rn = neighbor_rank(st, r, lo, hi, C);
+      }
+      mine[u] = rn;
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+#pragma unroll
+      for (int q = 0; q < GM; ++q) {
+        uint32_t v;
+        if constexpr (GM == 1) v = mine[u];
+        else if (q == 0) v = group_bcast<GM, 0>(mine[u]);
+        else if (q == 1) v = group_bcast<GM, 1>(mine[u]);
+        else if (q == 2) v = group_bcast<GM, 2>(mine[u]);
+        else if (q == 3) v = group_bcast<GM, 3>(mine[u]);
+        else if (q == 4) v = group_bcast<GM, 4>(mine[u]);
+        else if (q == 5) v = group_bcast<GM, 5>(mine[u]);
+        else if (q == 6) v = group_bcast<GM, 6>(mine[u]);
+        else v = group_bcast<GM, 7>(mine[u]);
+        out[q + GM * u] = v;
+      }
+    }
+  }
+}
+
 // out = cx*X + ch*((H - shift)*scale) b2 + cb*b3 on rows of the Lup sector.
-template <typename T, int V>
+// The G = s/V lanes of a row split the neighbour-rank work (GM = G) and each
+// gathers its V columns of every flippable neighbour; NBAT neighbours are
+// loaded before any is accumulated.
+template <typename T, int V, int GM>
 __global__ __launch_bounds__(256) void chain_step_kernel(
     const ChainParams P, const uint32_t* __restrict__ states, const uint32_t* __restrict__ binom,
     const double* __restrict__ diag, const Affine af, const int flags, const T cx, const T ch,
@@ -99,9 +187,14 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
     void* __restrict__ OUT, const Layout lay, const int64_t out_ld, double* __restrict__ dot_part,
     const T* __restrict__ Pr, const int64_t ngroups) {
   typedef typename VT<T, V>::t vt;
+  constexpr int GE = GM < 1 ? 1 : GM;
+  constexpr int NB0 = (V * (int)sizeof(T) <= 32) ? 8 : 4;
+  constexpr int NBAT = NB0 < GE ? GE : NB0;
   __shared__ uint32_t sC[32 * kRankT];
+  __shared__ T sW[kMaxBonds];
   __shared__ double red[4];
   for (int i = threadIdx.x; i < 32 * kRankT; i += blockDim.x) sC[i] = binom[i];
+  for (int i = threadIdx.x; i < P.nb; i += blockDim.x) sW[i] = (T)(P.bval[i] * af.scale);
   __syncthreads();
 
   const int rloc = threadIdx.x / lay.G;
@@ -111,27 +204,35 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
   const int out_col0 = (int)blockIdx.y + g * V;
   double dacc = 0.0;
 
-  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+  const bool remap = lay.xcd && (int64_t)gridDim.x == ngroups;
+  for (int64_t it = blockIdx.x; it < ngroups; it += gridDim.x) {
+    const int64_t grp = remap ? xcd_block(it, ngroups) : it;
     const int64_t row = grp * lay.rows_per_block + rloc;
-    if (!lane_ok || row >= P.D) continue;
+    if (!lane_ok || row >= P.D) continue;  // whole row groups drop out together
     vt h = (vt)(T)0;
     if (flags & kHasH) {
       const uint32_t st = states[row];
       const uint32_t r32 = (uint32_t)row;
+      const T* __restrict__ B2c = B2 + cofs;
+      const uint32_t rs32 = (uint32_t)lay.rs;  // D*rs < 2^32 is checked on the host
       vt tot = (vt)(T)0;
-      for (int c = 0; c < P.ncls; ++c) {
-        vt acc = (vt)(T)0;
-        for (int b = P.cls_begin[c]; b < P.cls_begin[c + 1]; ++b) {
-          const int lo = P.lo[b], hi = P.hi[b];
-          if (((st >> lo) ^ (st >> hi)) & 1u) {
-            const uint32_t r2 = neighbor_rank(st, r32, lo, hi, sC);
-            acc += vload<T, V>(B2 + (int64_t)r2 * lay.rs + cofs);
-          }
+      for (int b0 = 0; b0 < P.nb; b0 += NBAT) {
+        uint32_t rk[NBAT];
+        bond_ranks<GM, NBAT>(P, b0, st, r32, sC, g, rk);
+        vt x[NBAT];
+#pragma unroll
+        for (int j = 0; j < NBAT; ++j) {
+          x[j] = (vt)(T)0;
+          if (rk[j] != r32) x[j] = vload<T, V>(B2c + rk[j] * rs32);
         }
-        tot += (T)(P.cval[c] * af.scale) * acc;
+#pragma unroll
+        for (int j = 0; j < NBAT; ++j) {
+          const int b = b0 + j;
+          if (b < P.nb) tot += sW[b] * x[j];
+        }
       }
       const T d = (T)((diag[row] - af.shift) * af.scale);
-      h = d * vload<T, V>(B2 + row * lay.rs + cofs) + tot;
+      h = d * vload<T, V>(B2c + r32 * rs32) + tot;
     }
     step_epilogue<T, V>(flags, row, lay.rs, cofs, out_col0, cx, ch, cb, h, X, B3, OUT, out_ld, Pr,
                         dacc);
@@ -163,7 +264,9 @@ __global__ __launch_bounds__(256) void qsun_step_kernel(
   const int out_col0 = (int)blockIdx.y + g * V;
   double dacc = 0.0;
 
-  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+  const bool remap = lay.xcd && (int64_t)gridDim.x == ngroups;
+  for (int64_t it = blockIdx.x; it < ngroups; it += gridDim.x) {
+    const int64_t grp = remap ? xcd_block(it, ngroups) : it;
     const int64_t row = grp * lay.rows_per_block + rloc;
     if (!lane_ok || row >= P.D) continue;
     vt h = (vt)(T)0;
@@ -171,19 +274,34 @@ __global__ __launch_bounds__(256) void qsun_step_kernel(
       const uint32_t r = (uint32_t)row;
       const int a = (int)(r & (uint32_t)(dg - 1));
       const uint32_t base = r & ~(uint32_t)(dg - 1);
+      const T* __restrict__ B2c = B2 + cofs;
+      const uint32_t rs32 = (uint32_t)lay.rs;
       // diagonal in the reference order: outer fields, then the grain diagonal
       double dd = 0.0;
       for (int j = 0; j < P.Lloc; ++j) dd += P.h[j] * (((r >> (P.Lg + j)) & 1u) ? 0.5 : -0.5);
       dd += P.G[a * dg + a];
+      const T d = (T)((dd - af.shift) * af.scale);
+      // grain block: the whole dg-row group, the diagonal element weighted by d
       vt acc = (vt)(T)0;
       for (int b = 0; b < dg; ++b) {
-        if (b == a) continue;
-        acc += sG[a * dg + b] * vload<T, V>(B2 + (int64_t)(base | (uint32_t)b) * lay.rs + cofs);
+        const T wb = (b == a) ? d : sG[a * dg + b];
+        acc += wb * vload<T, V>(B2c + (base | (uint32_t)b) * rs32);
       }
-      for (int j = 0; j < P.Lloc; ++j)
-        acc += (T)(P.coup[j] * af.scale) * vload<T, V>(B2 + (int64_t)(r ^ P.flip[j]) * lay.rs + cofs);
-      const T d = (T)((dd - af.shift) * af.scale);
-      h = d * vload<T, V>(B2 + row * lay.rs + cofs) + acc;
+      // outer-spin flips (always present in the full space): batched gathers
+      for (int j0 = 0; j0 < P.Lloc; j0 += kGatherBatch) {
+        vt x[kGatherBatch];
+        T w[kGatherBatch];
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) {
+          const int j = j0 + u;
+          const bool ok = j < P.Lloc;  // uniform
+          w[u] = ok ? (T)(P.coup[j] * af.scale) : (T)0;
+          x[u] = vload<T, V>(B2c + (ok ? (r ^ P.flip[j]) : r) * rs32);
+        }
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) acc += w[u] * x[u];
+      }
+      h = acc;
     }
     step_epilogue<T, V>(flags, row, lay.rs, cofs, out_col0, cx, ch, cb, h, X, B3, OUT, out_ld, Pr,
                         dacc);
@@ -195,20 +313,39 @@ __global__ __launch_bounds__(256) void qsun_step_kernel(
 }
 
 // ------------------------------------------------------------ launching
-Layout interleaved_layout(int dtype, int s) {
+static bool valid_vec(int dtype, int s, int V) {
+  if (V < 1 || s % V != 0) return false;
+  if (dtype == PF_F64 && !(V == 1 || V == 2 || V == 4 || V == 8 || V == 16)) return false;
+  if (dtype == PF_F32 && !(V == 1 || V == 2 || V == 4 || V == 8 || V == 16 || V == 32))
+    return false;
+  return s / V <= 256;
+}
+
+Layout interleaved_layout(int dtype, int s, int V, int xcd) {
   Layout l{};
-  int V = 1;
-  if (dtype == PF_F64) {
-    V = (s % 2 == 0) ? 2 : 1;
-  } else {
-    V = (s % 4 == 0) ? 4 : (s % 2 == 0 ? 2 : 1);
+  if (V == 0 || !valid_vec(dtype, s, V)) {
+    if (dtype == PF_F64)
+      V = (s % 2 == 0) ? 2 : 1;
+    else
+      V = (s % 4 == 0) ? 4 : (s % 2 == 0 ? 2 : 1);
   }
   l.V = V;
   l.G = s / V;
   l.rs = s;
   l.cstride = 0;
   l.rows_per_block = 256 / l.G;
+  l.xcd = xcd;
   return l;
+}
+
+// Configuration for (dtype, s): forced knobs, else the autotuned entry, else defaults.
+Layout tuned_layout(pf_op* op, int dtype, int s) {
+  TuneCfg c;
+  auto it = op->tuned.find({dtype, s});
+  if (it != op->tuned.end()) c = it->second;
+  if (op->force.V) c.V = op->force.V;
+  if (op->force.xcd >= 0) c.xcd = op->force.xcd;
+  return interleaved_layout(dtype, s, c.V, c.xcd < 0 ? 1 : c.xcd);
 }
 
 Layout colmajor_layout(int64_t ld) {
@@ -218,6 +355,7 @@ Layout colmajor_layout(int64_t ld) {
   l.rs = 1;
   l.cstride = ld;
   l.rows_per_block = 256;
+  l.xcd = 1;
   return l;
 }
 
@@ -226,6 +364,8 @@ static void launch_typed(pf_op* op, int flags, const Layout& lay, int ncols_grid
                          double cx, double ch, double cb, const void* X, const void* B2,
                          const void* B3, void* out, int64_t out_ld, hipStream_t st,
                          double* dot_part, const void* P, int dot_blocks) {
+  PF_REQUIRE((uint64_t)op->D * (uint64_t)lay.rs < ((uint64_t)1 << 32),
+             "block too large for 32-bit row offsets");
   const int64_t ngroups = (op->D + lay.rows_per_block - 1) / lay.rows_per_block;
   int64_t gx = ngroups;
   if (flags & kDot) gx = std::min<int64_t>(ngroups, dot_blocks);
@@ -236,9 +376,18 @@ static void launch_typed(pf_op* op, int flags, const Layout& lay, int ncols_grid
   const T* B3t = reinterpret_cast<const T*>(B3);
   const T* Pt = reinterpret_cast<const T*>(P);
   if (op->kind == kChain) {
-    hipLaunchKernelGGL((chain_step_kernel<T, V>), grid, block, 0, st, op->chain, op->states,
-                       op->binom, op->diag, af, flags, (T)cx, (T)ch, (T)cb, Xt, B2t, B3t, out,
-                       lay, out_ld, dot_part, Pt, ngroups);
+#define PF_CHAIN_LAUNCH(GMV)                                                                   \
+  hipLaunchKernelGGL((chain_step_kernel<T, V, GMV>), grid, block, 0, st, op->chain, op->states, \
+                     op->binom, op->diag, af, flags, (T)cx, (T)ch, (T)cb, Xt, B2t, B3t, out,   \
+                     lay, out_ld, dot_part, Pt, ngroups)
+    switch (lay.G) {
+      case 1: PF_CHAIN_LAUNCH(1); break;
+      case 2: PF_CHAIN_LAUNCH(2); break;
+      case 4: PF_CHAIN_LAUNCH(4); break;
+      case 8: PF_CHAIN_LAUNCH(8); break;
+      default: PF_CHAIN_LAUNCH(0); break;
+    }
+#undef PF_CHAIN_LAUNCH
   } else {
     hipLaunchKernelGGL((qsun_step_kernel<T, V>), grid, block, 0, st, op->qsun, af, flags, (T)cx,
                        (T)ch, (T)cb, Xt, B2t, B3t, out, lay, out_ld, dot_part, Pt, ngroups);
@@ -256,6 +405,8 @@ void launch_step(pf_op* op, int dtype, int flags, const Layout& lay, int ncols, 
       case 1: launch_typed<double, 1>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
       case 2: launch_typed<double, 2>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
       case 4: launch_typed<double, 4>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
+      case 8: launch_typed<double, 8>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
+      case 16: launch_typed<double, 16>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
       default: throw Error{PF_ERR_UNSUPPORTED, "unsupported vector width"};
     }
   } else if (dtype == PF_F32) {
@@ -264,6 +415,8 @@ void launch_step(pf_op* op, int dtype, int flags, const Layout& lay, int ncols, 
       case 2: launch_typed<float, 2>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
       case 4: launch_typed<float, 4>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
       case 8: launch_typed<float, 8>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
+      case 16: launch_typed<float, 16>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
+      case 32: launch_typed<float, 32>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
       default: throw Error{PF_ERR_UNSUPPORTED, "unsupported vector width"};
     }
   } else {
@@ -520,10 +673,12 @@ extern "C" int pf_op_create_spin_chain(pf_handle* h, int32_t L, int32_t Lup, int
         if (halfj[b] == classes[c]) {
           P.lo[nb] = std::min(oi[b], oj[b]);
           P.hi[nb] = std::max(oi[b], oj[b]);
+          P.bval[nb] = classes[c];
           ++nb;
         }
     }
     P.cls_begin[P.ncls] = nb;
+    P.nb = nb;  // bonds with a nonzero off-diagonal value, in class order
 
     // basis (ascending, Gosper's hack) and binomial table
     std::vector<uint32_t> states(D);
@@ -769,7 +924,7 @@ extern "C" int pf_filter(pf_op* op, int32_t dtype, int32_t polytype, const doubl
     const int cmax = chunk_cols(dtype);
     for (int c0 = 0; c0 < ncols; c0 += cmax) {
       const int s = std::min(cmax, ncols - c0);
-      const Layout lay = interleaved_layout(dtype, s);
+      const Layout lay = tuned_layout(op, dtype, s);
       const size_t blk = (size_t)op->D * s * esz;
       char* base = reinterpret_cast<char*>(op->fbuf.get(4 * blk));
       void* Xp = base;
@@ -836,7 +991,7 @@ extern "C" int pf_kpm_moments(pf_op* op, int32_t dtype, int32_t N, int32_t R, co
     PF_HIP(hipMemsetAsync(mu_dev, 0, N * sizeof(double), st));
     for (int c0 = 0; c0 < R; c0 += cmax) {
       const int s = std::min(cmax, R - c0);
-      const Layout lay = interleaved_layout(dtype, s);
+      const Layout lay = tuned_layout(op, dtype, s);
       const size_t blk = (size_t)op->D * s * esz;
       char* base = reinterpret_cast<char*>(op->fbuf.get(4 * blk));
       void* P = base;  // normalized probes = T_0
@@ -876,5 +1031,80 @@ extern "C" int pf_kpm_moments(pf_op* op, int32_t dtype, int32_t N, int32_t R, co
     PF_HIP(hipStreamSynchronize(st));
     const double scale = (double)op->D / (double)R;
     for (int n = 0; n < N; ++n) mu_out[n] = op->khost[n] * scale;
+  });
+}
+
+extern "C" int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value) {
+  return guarded([&] {
+    PF_REQUIRE(op, "null operator");
+    if (key == PF_TUNE_XCD_REMAP) {
+      PF_REQUIRE(value >= -1 && value <= 1, "tuning: xcd remap is -1, 0 or 1");
+      op->force.xcd = value;
+    } else if (key == PF_TUNE_VEC) {
+      PF_REQUIRE(value == 0 || value == 1 || value == 2 || value == 4 || value == 8 ||
+                     value == 16 || value == 32,
+                 "tuning: vector width 0, 1, 2, 4, 8, 16 or 32");
+      op->force.V = value;
+    } else if (key == PF_TUNE_RESET) {
+      op->force = TuneCfg{};
+      op->tuned.clear();
+    } else {
+      throw Error{PF_ERR_INVALID, "tuning: unknown key"};
+    }
+  });
+}
+
+// Times the fused recurrence step for every candidate (vector width, XCD
+// mapping) on the operator's own shape and keeps the fastest for (dtype, ncols).
+extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nsteps,
+                           double* best_us, int32_t* best_vec, int32_t* best_xcd,
+                           pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(op, "null operator");
+    PF_REQUIRE(dtype == PF_F64 || dtype == PF_F32, "autotune: dtype");
+    PF_REQUIRE(ncols >= 1 && ncols <= chunk_cols(dtype), "autotune: 1 <= ncols <= chunk width");
+    PF_REQUIRE(nsteps >= 1 && nsteps <= 100000, "autotune: nsteps");
+    hipStream_t st = S(stream);
+    const size_t esz = dtype == PF_F64 ? 8 : 4;
+    const size_t blk = (size_t)op->D * ncols * esz;
+    char* base = reinterpret_cast<char*>(op->fbuf.get(4 * blk));
+    PF_HIP(hipMemsetAsync(base, 0, 4 * blk, st));
+    void* b[4] = {base, base + blk, base + 2 * blk, base + 3 * blk};
+    const Affine af = op->affine(true);
+    hipEvent_t e0, e1;
+    PF_HIP(hipEventCreate(&e0));
+    PF_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    TuneCfg bestc;
+    const int vs[6] = {1, 2, 4, 8, 16, 32};
+    for (int vi = 0; vi < 6; ++vi) {
+      if (!valid_vec(dtype, ncols, vs[vi])) continue;
+      for (int x = 0; x <= 1; ++x) {
+        const Layout lay = interleaved_layout(dtype, ncols, vs[vi], x);
+        for (int w = 0; w < 2; ++w)
+          launch_step(op, dtype, kHasX | kHasH | kHasB3, lay, ncols, af, 0.5, 2.0, -1.0, b[0],
+                      b[1], b[2], b[3], 0, st);
+        PF_HIP(hipEventRecord(e0, st));
+        for (int k = 0; k < nsteps; ++k)  // rotate three distinct recurrence buffers
+          launch_step(op, dtype, kHasX | kHasH | kHasB3, lay, ncols, af, 0.5, 2.0, -1.0, b[0],
+                      b[1 + k % 3], b[1 + (k + 1) % 3], b[1 + (k + 2) % 3], 0, st);
+        PF_HIP(hipEventRecord(e1, st));
+        PF_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double us = 1e3 * ms / nsteps;
+        if (us < best) {
+          best = us;
+          bestc.V = vs[vi];
+          bestc.xcd = x;
+        }
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    op->tuned[{dtype, ncols}] = bestc;
+    if (best_us) *best_us = best;
+    if (best_vec) *best_vec = bestc.V;
+    if (best_xcd) *best_xcd = bestc.xcd;
   });
 }

@@ -5,7 +5,9 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/polfed_hip.h"
@@ -91,6 +93,7 @@ struct ChainParams {
   int32_t cls_begin[kMaxClasses + 1];  // bonds grouped by off-diagonal value class
   int8_t lo[kMaxBonds], hi[kMaxBonds];  // 0-based bit positions, lo < hi
   double cval[kMaxClasses];             // unscaled off-diagonal value J/2 of each class
+  double bval[kMaxBonds];               // cval of each bond's class
 };
 
 // Quantum-Sun operator parameters (spin 1/2, full space).
@@ -126,6 +129,13 @@ struct Layout {
   int G;           // lanes per row
   int rows_per_block;
   int V;           // elements per lane
+  int xcd;         // 1: map consecutive row groups to the same XCD (L2 locality)
+};
+
+// Launch configuration of the row-interleaved step kernel for one (dtype, s).
+struct TuneCfg {
+  int V = 0;    // elements per lane (0 = default)
+  int xcd = -1; // XCD-contiguous row mapping (-1 = default)
 };
 
 }  // namespace pf
@@ -149,6 +159,8 @@ struct pf_op {
   uint32_t* binom = nullptr;   // spin chain: C(p, t), p < 32, t < kRankT
   double* diag = nullptr;      // spin chain: unscaled diagonal, reference summation order
   pf::DevBuf fbuf;             // filter / KPM row-interleaved work blocks
+  pf::TuneCfg force;           // pf_op_set_tuning overrides
+  std::map<std::pair<int, int>, pf::TuneCfg> tuned;  // autotuned (dtype, s) configs
   pf::DevBuf kpart;            // KPM partial dots
   std::vector<double> khost;   // KPM host staging
   pf::Affine affine(bool rescaled) const {
@@ -169,7 +181,8 @@ void launch_step(pf_op* op, int dtype, int flags, const Layout& lay, int ncols, 
                  void* out, int64_t out_ld, hipStream_t st, double* dot_part = nullptr,
                  const void* P = nullptr, int dot_blocks = 0);
 
-Layout interleaved_layout(int dtype, int s);
+Layout interleaved_layout(int dtype, int s, int V = 0, int xcd = 1);
+Layout tuned_layout(pf_op* op, int dtype, int s);
 Layout colmajor_layout(int64_t ld);
 
 // Row-interleaved pack: dst[r*s + c] = (T)(src[r + c*ld] * colscale[c]) (colscale may be null).

@@ -130,6 +130,26 @@ class Operator:
                                       self.device.stream))
         return mu
 
+    def set_tuning(self, xcd: int | None = None, vec: int | None = None, reset: bool = False):
+        """Force launch knobs of the row-interleaved kernels (see pf_op_set_tuning)."""
+        if reset:
+            check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_RESET, 0))
+        if xcd is not None:
+            check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_XCD_REMAP, int(xcd)))
+        if vec is not None:
+            check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_VEC, int(vec)))
+
+    def autotune(self, ncols: int, dtype="f64", nsteps: int = 20):
+        """Pick the fastest step-kernel launch configuration for (dtype, ncols)
+        on this operator; returns (us per step, vector width, xcd mapping)."""
+        code, _ = _dtype(dtype)
+        us = ctypes.c_double()
+        vec = ctypes.c_int32()
+        xcd = ctypes.c_int32()
+        check(self.lib.pf_autotune(self.op, code, int(ncols), int(nsteps), ctypes.byref(us),
+                                   ctypes.byref(vec), ctypes.byref(xcd), self.device.stream))
+        return us.value, vec.value, xcd.value
+
     def connectivity(self, row0: int = 0, nrows: int | None = None, max_per_row: int = 64):
         """Per-row off-diagonal (cols, vals, counts) and the unscaled diagonal,
         as computed by the device kernels (host numpy arrays)."""

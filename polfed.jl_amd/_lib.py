@@ -29,6 +29,7 @@ PF_TAYLOR = 3
 PF_TUNE_XCD_REMAP = 1
 PF_TUNE_VEC = 2
 PF_TUNE_RESET = 3
+PF_TUNE_NONTEMPORAL = 4
 
 PF_PROBE_GAUSSIAN = 0
 PF_PROBE_PHASE = 1

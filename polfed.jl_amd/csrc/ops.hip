@@ -48,6 +48,16 @@ template <typename T, int V>
 __device__ __forceinline__ void vstore(T* p, typename VT<T, V>::t v) {
   *reinterpret_cast<typename VT<T, V>::t*>(p) = v;
 }
+// Streamed (read-once / write-once) accesses with the non-temporal hint, so
+// they do not evict the gathered rows from L2.
+template <typename T, int V>
+__device__ __forceinline__ typename VT<T, V>::t vload_nt(const T* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const typename VT<T, V>::t*>(p));
+}
+template <typename T, int V>
+__device__ __forceinline__ void vstore_nt(T* p, typename VT<T, V>::t v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<typename VT<T, V>::t*>(p));
+}
 template <typename T, int V>
 __device__ __forceinline__ T vget(typename VT<T, V>::t v, int i) {
   if constexpr (V == 1) {
@@ -88,13 +98,16 @@ __device__ __forceinline__ void step_epilogue(int flags, int64_t row, int64_t rs
   typedef typename VT<T, V>::t vt;
   const int64_t off = row * rs + cofs;
   vt o = (vt)(T)0;
-  if (flags & kHasX) o = cx * vload<T, V>(X + off);
+  const bool nt = flags & kStreamNT;
+  if (flags & kHasX) o = cx * (nt ? vload_nt<T, V>(X + off) : vload<T, V>(X + off));
   if (flags & kHasH) o += ch * h;
-  if (flags & kHasB3) o += cb * vload<T, V>(B3 + off);
+  if (flags & kHasB3) o += cb * (nt ? vload_nt<T, V>(B3 + off) : vload<T, V>(B3 + off));
   if (flags & kOutCM) {
     double* Y = reinterpret_cast<double*>(OUT);
 #pragma unroll
     for (int v = 0; v < V; ++v) Y[row + (int64_t)(out_col0 + v) * out_ld] = (double)vget<T, V>(o, v);
+  } else if (nt) {
+    vstore_nt<T, V>(reinterpret_cast<T*>(OUT) + off, o);
   } else {
     vstore<T, V>(reinterpret_cast<T*>(OUT) + off, o);
   }
@@ -211,7 +224,9 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
     if (!lane_ok || row >= P.D) continue;  // whole row groups drop out together
     vt h = (vt)(T)0;
     if (flags & kHasH) {
-      const uint32_t st = states[row];
+      const bool nt = flags & kStreamNT;
+      const uint32_t st = nt ? __builtin_nontemporal_load(states + row) : states[row];
+      const double drow = nt ? __builtin_nontemporal_load(diag + row) : diag[row];
       const uint32_t r32 = (uint32_t)row;
       const T* __restrict__ B2c = B2 + cofs;
       const uint32_t rs32 = (uint32_t)lay.rs;  // D*rs < 2^32 is checked on the host
@@ -231,7 +246,7 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
           if (b < P.nb) tot += sW[b] * x[j];
         }
       }
-      const T d = (T)((diag[row] - af.shift) * af.scale);
+      const T d = (T)((drow - af.shift) * af.scale);
       h = d * vload<T, V>(B2c + r32 * rs32) + tot;
     }
     step_epilogue<T, V>(flags, row, lay.rs, cofs, out_col0, cx, ch, cb, h, X, B3, OUT, out_ld, Pr,
@@ -321,7 +336,7 @@ static bool valid_vec(int dtype, int s, int V) {
   return s / V <= 256;
 }
 
-Layout interleaved_layout(int dtype, int s, int V, int xcd) {
+Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt) {
   Layout l{};
   if (V == 0 || !valid_vec(dtype, s, V)) {
     if (dtype == PF_F64)
@@ -335,6 +350,7 @@ Layout interleaved_layout(int dtype, int s, int V, int xcd) {
   l.cstride = 0;
   l.rows_per_block = 256 / l.G;
   l.xcd = xcd;
+  l.nt = nt;
   return l;
 }
 
@@ -345,7 +361,8 @@ Layout tuned_layout(pf_op* op, int dtype, int s) {
   if (it != op->tuned.end()) c = it->second;
   if (op->force.V) c.V = op->force.V;
   if (op->force.xcd >= 0) c.xcd = op->force.xcd;
-  return interleaved_layout(dtype, s, c.V, c.xcd < 0 ? 1 : c.xcd);
+  if (op->force.nt >= 0) c.nt = op->force.nt;
+  return interleaved_layout(dtype, s, c.V, c.xcd < 0 ? 1 : c.xcd, c.nt < 0 ? 1 : c.nt);
 }
 
 Layout colmajor_layout(int64_t ld) {
@@ -356,6 +373,7 @@ Layout colmajor_layout(int64_t ld) {
   l.cstride = ld;
   l.rows_per_block = 256;
   l.xcd = 1;
+  l.nt = 0;
   return l;
 }
 
@@ -371,6 +389,7 @@ static void launch_typed(pf_op* op, int flags, const Layout& lay, int ncols_grid
   if (flags & kDot) gx = std::min<int64_t>(ngroups, dot_blocks);
   PF_REQUIRE(gx < (int64_t)1 << 31, "grid too large");
   dim3 grid((unsigned)gx, (unsigned)ncols_grid), block(256);
+  if (lay.nt) flags |= kStreamNT;
   const T* Xt = reinterpret_cast<const T*>(X);
   const T* B2t = reinterpret_cast<const T*>(B2);
   const T* B3t = reinterpret_cast<const T*>(B3);
@@ -1045,6 +1064,9 @@ extern "C" int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value) {
                      value == 16 || value == 32,
                  "tuning: vector width 0, 1, 2, 4, 8, 16 or 32");
       op->force.V = value;
+    } else if (key == PF_TUNE_NONTEMPORAL) {
+      PF_REQUIRE(value >= -1 && value <= 1, "tuning: non-temporal is -1, 0 or 1");
+      op->force.nt = value;
     } else if (key == PF_TUNE_RESET) {
       op->force = TuneCfg{};
       op->tuned.clear();
@@ -1079,8 +1101,9 @@ extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nste
     const int vs[6] = {1, 2, 4, 8, 16, 32};
     for (int vi = 0; vi < 6; ++vi) {
       if (!valid_vec(dtype, ncols, vs[vi])) continue;
-      for (int x = 0; x <= 1; ++x) {
-        const Layout lay = interleaved_layout(dtype, ncols, vs[vi], x);
+      for (int cfg = 0; cfg < 4; ++cfg) {
+        const int x = cfg & 1, ntv = cfg >> 1;
+        const Layout lay = interleaved_layout(dtype, ncols, vs[vi], x, ntv);
         for (int w = 0; w < 2; ++w)
           launch_step(op, dtype, kHasX | kHasH | kHasB3, lay, ncols, af, 0.5, 2.0, -1.0, b[0],
                       b[1], b[2], b[3], 0, st);
@@ -1097,6 +1120,7 @@ extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nste
           best = us;
           bestc.V = vs[vi];
           bestc.xcd = x;
+          bestc.nt = ntv;
         }
       }
     }

@@ -121,6 +121,7 @@ enum : int {
   kHasB3 = 4,   // read b3
   kOutCM = 8,   // write the output column-major (FP64) instead of interleaved
   kDot = 16,    // accumulate <P, out> per block (KPM)
+  kStreamNT = 32,  // non-temporal hints on the streamed vectors
 };
 
 struct Layout {
@@ -130,12 +131,14 @@ struct Layout {
   int rows_per_block;
   int V;           // elements per lane
   int xcd;         // 1: map consecutive row groups to the same XCD (L2 locality)
+  int nt;          // 1: non-temporal hints on streamed loads/stores
 };
 
 // Launch configuration of the row-interleaved step kernel for one (dtype, s).
 struct TuneCfg {
   int V = 0;    // elements per lane (0 = default)
   int xcd = -1; // XCD-contiguous row mapping (-1 = default)
+  int nt = -1;  // non-temporal streamed accesses (-1 = default)
 };
 
 }  // namespace pf
@@ -181,7 +184,7 @@ void launch_step(pf_op* op, int dtype, int flags, const Layout& lay, int ncols, 
                  void* out, int64_t out_ld, hipStream_t st, double* dot_part = nullptr,
                  const void* P = nullptr, int dot_blocks = 0);
 
-Layout interleaved_layout(int dtype, int s, int V = 0, int xcd = 1);
+Layout interleaved_layout(int dtype, int s, int V = 0, int xcd = 1, int nt = 1);
 Layout tuned_layout(pf_op* op, int dtype, int s);
 Layout colmajor_layout(int64_t ld);
 

@@ -130,7 +130,8 @@ class Operator:
                                       self.device.stream))
         return mu
 
-    def set_tuning(self, xcd: int | None = None, vec: int | None = None, reset: bool = False):
+    def set_tuning(self, xcd: int | None = None, vec: int | None = None, nt: int | None = None,
+                   reset: bool = False):
         """Force launch knobs of the row-interleaved kernels (see pf_op_set_tuning)."""
         if reset:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_RESET, 0))
@@ -138,6 +139,8 @@ class Operator:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_XCD_REMAP, int(xcd)))
         if vec is not None:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_VEC, int(vec)))
+        if nt is not None:
+            check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_NONTEMPORAL, int(nt)))
 
     def autotune(self, ncols: int, dtype="f64", nsteps: int = 20):
         """Pick the fastest step-kernel launch configuration for (dtype, ncols)

@@ -34,3 +34,8 @@ def pytest_configure(config):
 def dev():
     from polfed_amd.device import Device
     return Device(0)
+
+
+def pytest_collection_modifyitems(config, items):
+    """Run CPU tests first; GPU tests keep their file order."""
+    items.sort(key=lambda it: 1 if it.get_closest_marker("gpu") else 0)

@@ -1,0 +1,123 @@
+"""CPU: the C ABI library loads and exports every declared symbol; the host
+mirror's scalar logic (target specs, KPM kernels and DoS, bounds, order,
+coefficients, configs) agrees with the oracle.  No GPU calls."""
+import ctypes
+import json
+import pathlib
+import re
+
+import numpy as np
+import pytest
+
+import polfed_amd
+from oracle import clenshaw as oc
+from oracle import dos as odos
+from oracle import spectral as osp
+from polfed_amd import _lib
+from polfed_amd import spectral as spc
+from polfed_amd.configs import PolfedDefaults, TransformConfig
+
+REPO = pathlib.Path(__file__).resolve().parent.parent
+KAT = json.loads((REPO / "tests" / "golden" / "reference_kat.json").read_text())
+
+
+def _declared_symbols():
+    names = set()
+    for h in (REPO / "include").glob("*.h"):
+        text = h.read_text()
+        names |= set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(pf_\w+)\s*\(", text, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    declared = _declared_symbols()
+    assert len(declared) >= 20
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    for name in declared:
+        assert hasattr(lib, name), name
+    # every declared function has a ctypes signature in the binding
+    assert declared <= set(_lib.SIGNATURES)
+    lib2 = _lib.load()
+    assert lib2.pf_version() >= 100
+    assert lib2.pf_last_error() == b""
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    with pytest.raises(polfed_amd.PolfedHIPError):
+        _lib.load(tmp_path / "nope.so")
+
+
+def test_target_specs_host_mirror():
+    k = KAT["target_specs"]
+    for inp, kind, val in k["valid"]:
+        spec = spc.normalize_target_spec(tuple(inp) if isinstance(inp, list) else inp)
+        assert spec.kind == kind and spec.value == val
+    for inp in k["invalid"]:
+        with pytest.raises(ValueError):
+            spc.normalize_target_spec(tuple(inp) if isinstance(inp, list) else inp)
+
+
+def test_kernels_and_dos_host_mirror():
+    for name, fn in odos.KERNELS.items():
+        for n, N in [(0, 10), (3, 10), (7, 11), (249, 250)]:
+            assert abs(float(spc.get_kernel(name)(np.array(n), N)) - fn(n, N)) < 1e-12, name
+    mu = np.random.default_rng(0).standard_normal(64) * 3 + 50
+    rho_o = odos.dos_function(mu, "Jackson")
+    rho_h = spc.DensityOfStates(mu, "Jackson")
+    xs = np.linspace(-0.95, 0.95, 31)
+    assert np.max(np.abs(rho_h(xs) - np.array([rho_o(x) for x in xs]))) < 1e-9 * np.max(np.abs(rho_h(xs)))
+    k = KAT["dos_rho0_identity_dirichlet"]
+    assert abs(spc.DensityOfStates(np.full(k["N"], 4.0), "Dirichlet")(0.0) - k["expected_rho0"]) < 1e-12
+
+
+@pytest.mark.parametrize("polytype", ["Chebyshev", "Legendre", "Hermite", "Taylor"])
+def test_series_value_matches_oracle_clenshaw(polytype):
+    c = np.random.default_rng(3).standard_normal(25)
+    for x in (-0.8, 0.1, 0.65):
+        ref = oc.clenshaw_scalar(lambda n: c[n], len(c) - 1, x, polytype)
+        assert abs(float(spc.series_value(c, x, polytype)) - ref) < 1e-11 * max(1, abs(ref))
+
+
+def test_normalized_coefficients_match_oracle():
+    for t, K in [(0.0, 12), (0.37, 400), (-0.9, 3000)]:
+        a = spc.normalized_coefficients(PolfedDefaults.coefficients, t, K)
+        b = oc.filter_coefficients(t, K)
+        assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(b))
+
+
+def test_bounds_and_order_host_mirror_match_oracle():
+    # flat DoS known answer
+    plan = spc.build_transform_plan(TransformConfig(), 1.0, 0.0, 1, "middle")
+    spc.getbounds_from_dos(plan, lambda x: 1.0)
+    assert abs(plan.left + 0.5) < 1e-7 and abs(plan.right - 0.5) < 1e-7
+    # a realistic DoS: compare the full interval/order resolution with the oracle
+    mu = odos.exact_moments(np.cos(np.linspace(0.2, 2.9, 3000)), 250)
+    rho_h = spc.DensityOfStates(mu, "Jackson")
+    rho_o = odos.dos_function(mu, "Jackson")
+    for target, howmany in [(("rescaled", 0.0), 50), (("rescaled", 0.4), 100), ("maxdos", 20)]:
+        ph = spc.build_transform_plan(TransformConfig(), 1.0, 0.0, howmany, target)
+        spc.getbounds(ph, 1.0, 0.0, rho_h)
+        spc.getorderofexpansion(ph)
+        po = osp.build_transform_plan(1.0, 0.0, howmany, target)
+        osp.getspectraltransform(po, 1.0, 0.0, rho_o)
+        assert abs(ph.target - po.target) < 1e-12
+        assert abs(ph.order - po.order) <= 1, (target, ph.order, po.order)
+        if ph.order == po.order:
+            assert abs(ph.left - po.left) < 1e-7 and abs(ph.right - po.right) < 1e-7
+    # over-constrained transform raises
+    bad = spc.build_transform_plan(TransformConfig(left=-0.2, right=0.2, order=12), 1.0, 0.0, 1,
+                                   "middle")
+    with pytest.raises(RuntimeError):
+        spc.getbounds(bad, 1.0, 0.0, rho_h)
+
+
+def test_defaults_and_configs():
+    for howmany, s, eta, exp in KAT["expectedkrylovdim"]["cases"]:
+        assert PolfedDefaults.expectedkrylovdim(howmany, s, eta) == exp
+    assert PolfedDefaults.coefficients(0.3, 0) == 1.0
+    assert abs(PolfedDefaults.coefficients(0.3, 4) - 2 * np.cos(4 * np.arccos(0.3))) < 1e-15
+    fc = polfed_amd.FactorizationConfig()
+    assert fc.tol == 1e-14 and fc.eigentol == 1e-9 and fc.which == "LR"
+    assert isinstance(fc.rot, polfed_amd.FullRO) and fc.rot.passes == 2
+    d = polfed_amd.DoSConfig()
+    assert (d.N, d.R, d.kernel) == (250, 300, "Jackson")

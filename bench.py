@@ -142,7 +142,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--fp32", action="store_true", help="also time the FP32-filter path")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="skip the FP32-filter solves (mixed and pure FP32)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "single-GPU build: only --gpus 1 is supported"}))
@@ -199,18 +200,30 @@ def main():
                      "per_launch_bytes": bytes_step, "avg_launch_ms": ms_step,
                      "filter_share": filter_ms * 1e-3 / elapsed},
     }
-    if args.fp32:
-        timer.reset()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        v32, _, r32 = solve(op, x0, "f32")
-        torch.cuda.synchronize()
-        e32 = time.perf_counter() - t1
-        ms32, by32, _, _ = timer.summary(op.D, 4)
-        out["fp32_filter"] = {"value": e32, "eigenconverged": r32.eigenconverged,
-                              "max_residual": r32.eigenresidual,
-                              "speedup_vs_fp64": (elapsed / args.steps) / e32,
-                              "filter_GBps": by32 / (ms32 * 1e-3) / 1e9}
+    if not args.no_fp32:
+        import numpy as np
+        width = rep.Emax - rep.Emin
+        ref_vals = np.sort(vals)
+        for mode in ("mixed", "f32"):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            vm, _, rm = solve(op, x0, mode)
+            torch.cuda.synchronize()
+            em = time.perf_counter() - t1
+            lz = rm.lanczos
+            allv = np.sort(lz.all_vals) if lz.all_vals is not None else np.zeros(0)
+            n = min(len(allv), len(ref_vals))
+            entry = {
+                "time_s": em, "speedup_vs_fp64": (elapsed / args.steps) / em,
+                "returned_pairs": int(len(vm)), "lanczos_converged": int(lz.converged),
+                "max_residual_all": float(lz.all_residuals.max()) if lz.all_residuals is not None else None,
+                "max_eig_err_vs_fp64_over_width": float(np.max(np.abs(allv[:n] - ref_vals[:n])) / width) if n else None,
+                "iterations": int(lz.numiter), "fp32_iterations": int(lz.fast_iterations),
+                "switch_iteration": int(lz.switch_iteration), "fp64_fallback": bool(lz.fallback),
+            }
+            out[f"fp32_filter_{mode}"] = entry
+            log(f"{mode}: {em:.2f} s pairs={len(vm)} fp32 iters={lz.fast_iterations}/{lz.numiter} "
+                f"maxres={entry['max_residual_all']}")
     if not args.no_cpu:
         q = dev.to_host(x0)
         out["cpu_baseline"] = cpu_baseline(op, rep, q)

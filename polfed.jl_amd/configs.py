@@ -107,7 +107,11 @@ class MappingConfig:
 @dataclass
 class TransformConfig:
     """Polynomial filter options (TransformConfig.jl:6-100).  ``dtype``
-    selects the filter arithmetic on the GPU: "f64" (plain drop-in) or "f32"."""
+    selects the filter arithmetic on the GPU: "f64" (plain drop-in), "f32"
+    (every filter in FP32: eigenvalues at FP64 accuracy, residuals limited by
+    FP32 rounding), or "mixed" (FP64 until the wanted Ritz pairs' residual
+    estimate drops below ``switch_tol``, then FP32, falling back to FP64 if the
+    final residuals miss eigentol)."""
 
     coefficients: Callable = PolfedDefaults.coefficients
     normalization: float = PolfedDefaults.normalization
@@ -118,6 +122,7 @@ class TransformConfig:
     order_safety_factor: float = PolfedDefaults.order_safety_factor
     polynomialtype: str = PolfedDefaults.polynomialtype
     dtype: str = "f64"
+    switch_tol: float = 1e-6
 
 
 @dataclass

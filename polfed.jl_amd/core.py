@@ -210,13 +210,23 @@ def polfed(op: Operator, x0, howmany: int, target, *, produce_report: bool = Fal
     coeffs = spc.normalized_coefficients(plan.coefficients, plan.target, plan.order,
                                          plan.normalization, plan.polynomialtype)
     times["transform"] = time.perf_counter() - t0
-    f_t = _filter_callable(op, mapping, coeffs, plan.polynomialtype, transform.dtype)
+    if transform.dtype not in ("f64", "f32", "mixed"):
+        raise ValueError(f"TransformConfig.dtype must be 'f64', 'f32' or 'mixed', got {transform.dtype!r}")
+    f_t = _filter_callable(op, mapping, coeffs, plan.polynomialtype, "f64")
+    f_fast = None
+    if transform.dtype != "f64":
+        f_fast = _filter_callable(op, mapping, coeffs, plan.polynomialtype, "f32")
+    fast_mode = {"f64": "never", "f32": "always", "mixed": "mixed"}[transform.dtype]
+    # launch autotuning of the step kernel for this block width (cached per operator)
+    for dt in (("f64",) if transform.dtype == "f64" else ("f64", "f32")):
+        op.ensure_tuned(min(s, 32 if dt == "f64" else 64), dt)
 
     # ---- lanczos on the filtered operator (polfed_algorithm.jl:25-34)
     t0 = time.perf_counter()
     res = lanczos(dev, f_t, x0, howmany, maxdim=maxdim, which=fact.which, tol=fact.tol,
                   eigentol=fact.eigentol if fact.eigentol is not None else 1e-8, mapvals=op,
-                  rot=fact.rot)
+                  rot=fact.rot, f_fast=f_fast, fast_mode=fast_mode,
+                  switch_tol=transform.switch_tol)
     torch.cuda.synchronize()
     times["lanczos"] = time.perf_counter() - t0
     times.update({f"lanczos.{k}": v for k, v in res.times.items()})

@@ -47,6 +47,11 @@ class LanczosResult:
     krylovdim_history: list = field(default_factory=list)
     maxresidual_history: list = field(default_factory=list)
     times: dict = field(default_factory=dict)
+    all_vals: np.ndarray = None       # every Lanczos-converged pair (before the eigentol cut)
+    all_residuals: np.ndarray = None
+    fast_iterations: int = 0          # iterations whose filter ran in the fast (FP32) form
+    switch_iteration: int = -1        # iteration at which the mixed mode switched to FP32
+    fallback: bool = False            # mixed mode went back to FP64 after a failed residual check
 
 
 def sortvals(vals: np.ndarray, which: str) -> np.ndarray:
@@ -109,13 +114,22 @@ class _Timer:
 
 def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | None = None,
             which: str = "SR", tol: float = 1e-14, eigentol: float = 1e-8, mapvals=None,
-            rot=None, stage_timers: bool = True) -> LanczosResult:
+            rot=None, stage_timers: bool = True, f_fast=None, fast_mode: str = "never",
+            switch_tol: float = 1e-6) -> LanczosResult:
     """Block Lanczos for ``howmany`` pairs of the operator ``f``.
 
     ``f(X, Y)`` writes the operator applied to the (s, D) block X into Y.
     ``mapvals`` (default ``f``'s operator) evaluates the final Rayleigh
     quotients and residuals: an ``Operator`` (unscaled H) or a callable
     ``mapvals(X, Y)``.
+
+    ``f_fast`` is a cheaper, less accurate version of ``f`` (the FP32 filter).
+    ``fast_mode`` "always" uses it for every iteration; "mixed" starts with
+    ``f`` and switches to ``f_fast`` once the residual estimate
+    ||R_{k+1} phi_last|| of the ``howmany`` wanted Ritz pairs drops below
+    ``switch_tol`` (inexact-Krylov relaxation: late products may be inexact,
+    early ones may not), and returns to ``f`` if the final residuals miss
+    ``eigentol``.
     """
     rot = rot if rot is not None else FullRO()
     if maxdim is None:
@@ -152,6 +166,11 @@ def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | Non
     vecs = torch.empty((0, D), dtype=torch.float64, device=X0.device)
     res_out = np.zeros(0)
 
+    use_fast = fast_mode == "always" and f_fast is not None
+    fast_iters = 0
+    switch_it = -1
+    fallback = False
+    all_vals = all_res = None
     while True:
         # [1] tridiagonalization!: QR of r into the next basis slot
         with tm("tridiagonalization"):
@@ -164,10 +183,32 @@ def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | Non
             mat[k:k + s, k - s:k] = R
             mat[k - s:k, k:k + s] = R.T
             k += s
+        if fast_mode == "mixed" and f_fast is not None and not use_fast and not fallback \
+                and k - s >= howmany:
+            with tm("switch_test"):
+                kp = k - s
+                if which == "LR":
+                    sub = [kp - howmany, kp - 1]
+                elif which == "SR":
+                    sub = [0, howmany - 1]
+                else:
+                    sub = None
+                lam_p, ph_p = sla.eigh(mat[:kp, :kp], lower=False, check_finite=False,
+                                       subset_by_index=sub)
+                if sub is None:
+                    ph_p = ph_p[:, sortvals(lam_p, which)[:howmany]]
+                est = float(np.max(np.linalg.norm(R @ ph_p[kp - s:kp, :], axis=0)))
+                if est < switch_tol:
+                    use_fast = True
+                    switch_it = numiter + 1
         V_last, V_prev = basis[k - s:k], basis[k - 2 * s:k - s]
         # [2] mapping!
         with tm("mapping"):
-            f(V_last, r)
+            if use_fast:
+                f_fast(V_last, r)
+                fast_iters += 1
+            else:
+                f(V_last, r)
         # [3] calculate_overlap!  [4] orthogonalization!
         with tm("orthogonalization"):
             a = sym_upper(overlap(V_last, r))
@@ -220,7 +261,15 @@ def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | Non
                         else:
                             break
                     eigenresidual = float(res_out.max()) if res_out.size else 0.0
+                    all_vals, all_res = vals.copy(), res_out.copy()
                 nextcheck = _next_check(numiter, residual)
+        if converged == howmany and fast_mode == "mixed" and use_fast \
+                and eigenconverged < converged and numiter < maxiter:
+            # the fast products left residuals above eigentol: continue in FP64
+            use_fast = False
+            fallback = True
+            converged = 0
+            continue
         if converged == howmany or numiter >= maxiter:
             break
 
@@ -230,7 +279,8 @@ def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | Non
     vecs_sorted = vecs[:nconv].index_select(0, idx) if nconv else vecs[:0]
     return LanczosResult(vals[:nconv][order], vecs_sorted, res_out[:nconv][order], converged,
                          eigenconverged, numiter, maxiter, k, residual, eigenresidual, hist_c,
-                         hist_k, hist_r, dict(tm.t))
+                         hist_k, hist_r, dict(tm.t), all_vals, all_res, fast_iters, switch_it,
+                         fallback)
 
 
 def _eigen_residuals(dev: Device, mapvals, vecs: torch.Tensor):

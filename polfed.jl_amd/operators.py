@@ -153,6 +153,13 @@ class Operator:
                                    ctypes.byref(vec), ctypes.byref(xcd), self.device.stream))
         return us.value, vec.value, xcd.value
 
+    def ensure_tuned(self, ncols: int, dtype="f64", nsteps: int = 20):
+        """Autotune (dtype, ncols) once per operator."""
+        done = self.__dict__.setdefault("_tuned", {})
+        if (dtype, ncols) not in done:
+            done[(dtype, ncols)] = self.autotune(ncols, dtype, nsteps)
+        return done[(dtype, ncols)]
+
     def connectivity(self, row0: int = 0, nrows: int | None = None, max_per_row: int = 64):
         """Per-row off-diagonal (cols, vals, counts) and the unscaled diagonal,
         as computed by the device kernels (host numpy arrays)."""

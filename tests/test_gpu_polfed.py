@@ -113,3 +113,39 @@ def test_polfed_qsun_block(dev):
     assert np.max(np.abs(np.sort(vals) - exp)) <= 1e-10 * width
     psi = dev.to_host(vecs)
     assert np.linalg.norm(H @ psi - psi * vals, axis=0).max() <= 1e-10
+
+
+def test_polfed_mixed_precision_meets_residual_tolerance(dev):
+    """FP64 filter until the wanted Ritz pairs' residual estimate is small,
+    then FP32: same eigenvalues as the FP64 run (<= 1e-10 x width), every
+    residual <= 1e-10, and a nonzero number of FP32 iterations.  The pure
+    FP32 filter reproduces the eigenvalues but not the residual level."""
+    from polfed_amd import FactorizationConfig, MappingConfig, TransformConfig, polfed
+    from polfed_amd.blockops import fill_probes
+    from polfed_amd.models import xxz_hamiltonian
+    L, nev, s = 14, 24, 4
+    h = om.resolve_fields(L, 0.0, 5.0, rng=np.random.default_rng(2))
+    op = xxz_hamiltonian(L, L // 2, 1.0, 1.0, 5.0, fields=h, device=dev)
+    H = om.xxz_hamiltonian(L, L // 2, 1.0, 1.0, 5.0, fields=h)
+    ev = np.linalg.eigvalsh(H.toarray())
+    width = ev[-1] - ev[0]
+    q, _ = np.linalg.qr(np.random.default_rng(7).standard_normal((op.D, s)))
+    P = fill_probes(dev, op.D, 100, seed=5)
+    common = dict(produce_report=True, mapping=MappingConfig(Emin=ev[0], Emax=ev[-1]),
+                  fact=FactorizationConfig(eigentol=1e-10), probes=P)
+    v64, _, r64 = polfed(op, q, nev, "middle", transform=TransformConfig(dtype="f64"), **common)
+    vmx, psi, rmx = polfed(op, q, nev, "middle", transform=TransformConfig(dtype="mixed"), **common)
+    assert len(v64) >= nev - 4 and len(vmx) == len(v64)
+    assert rmx.lanczos.fast_iterations > 0
+    assert np.max(np.abs(vmx - v64)) <= 1e-10 * width
+    psi = dev.to_host(psi)
+    assert np.linalg.norm(H @ psi - psi * vmx, axis=0).max() <= 1e-10
+    v32, _, r32 = polfed(op, q, nev, "middle", transform=TransformConfig(dtype="f32"), **common)
+    lz = r32.lanczos
+    assert lz.fast_iterations == lz.numiter - 1
+    allv = np.sort(lz.all_vals)
+    assert len(allv) >= len(v64)
+    # every FP64 eigenvalue is reproduced by an FP32-filter Ritz value; the FP32
+    # operator coefficients perturb H~ by ~eps32, so agreement is ~1e-8 x width
+    assert max(np.min(np.abs(allv - v)) for v in v64) <= 1e-6 * width
+    assert lz.all_residuals.max() < 1e-3

@@ -28,6 +28,7 @@ if str(REPO) not in sys.path:
 
 METRIC = "time to nev eigenpairs (res≤1e-10), FP32-filter vs FP64; filter HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = REPO / "profiles" / "pmc_traffic.json"
 
 
 def _load_package():
@@ -93,6 +94,101 @@ def build_workload(dev, fp="f64"):
     return op, x0
 
 
+def kernel_name(op, dtype="f64", s=8):
+    """Template instance of the step kernel the autotuner picked for (dtype, s)."""
+    tuned = getattr(op, "_tuned", {}).get((dtype, s))
+    if tuned is None:
+        return None
+    V = tuned[1]
+    return f"chain_step_kernel<{'double' if dtype == 'f64' else 'float'}, {V}, {s // V}>"
+
+
+def measured_traffic(kernel):
+    """HBM-side bytes per launch of `kernel` on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter scaled by
+    its plain-copy calibration factor); None when that kernel was not counted."""
+    try:
+        table = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    entry = table.get("kernels", {}).get(kernel or "")
+    return None if entry is None else float(entry["bytes_per_launch"])
+
+
+def filter_only(args):
+    """Profiling driver: the bench workload's operator and start block, the
+    autotuned step kernel, `--filter-only` Clenshaw steps of a fixed order-200
+    filter; prints one JSON line with the average step time (HIP events)."""
+    import numpy as np
+    import torch
+    from polfed_amd.device import Device
+    dev = Device(0)
+    op, x0 = build_workload(dev)
+    op.set_rescale(-17.2, 13.5)
+    dt = args.dtype
+    op.ensure_tuned(8, dt)
+    c = np.cos(np.arange(201) * 0.3) * 0.01
+    y = torch.empty_like(x0)
+    reps = max(1, args.filter_only // 201)
+    op.filter(c, x0, y, dtype=dt)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        op.filter(c, x0, y, dtype=dt)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / (reps * 201)
+    esz = 8 if dt == "f64" else 4
+    print(json.dumps({"mode": "filter-only", "kernel": kernel_name(op, dt), "dtype": dt,
+                      "D": op.D, "s": 8, "steps": reps * 201, "us_per_step": us,
+                      "algorithmic_GBps": 4 * 8 * op.D * esz / (us * 1e-6) / 1e9}), flush=True)
+
+
+def reorth_only(args):
+    """Profiling driver: CGS2 reorthogonalization W -= V (V^T W), twice, of an
+    8-column block against k=416 basis vectors at D=184756 (the basis size the
+    bench solve reaches), timed with HIP events; prints FP64 MFMA TFLOP/s and
+    algorithmic GB/s (per pass: 4 D k s flops, 8 (2 D k + 3 D s) bytes)."""
+    import torch
+    from polfed_amd import blockops as bo
+    from polfed_amd.device import Device
+    dev = Device(0)
+    D, k, s = 184756, 416, 8
+    g = torch.Generator(device="cuda").manual_seed(7)
+    V = torch.randn((k, D), dtype=torch.float64, device="cuda", generator=g)
+    W = torch.randn((s, D), dtype=torch.float64, device="cuda", generator=g)
+    reps = max(1, args.reorth_only)
+    bo.reorth(dev, V, W, passes=2)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        bo.reorth(dev, V, W, passes=2)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    flops = 2 * 4.0 * D * k * s
+    nbytes = 2 * 8.0 * (2 * D * k + 3 * D * s)
+    print(json.dumps({"mode": "reorth-only", "D": D, "k": k, "s": s, "passes": 2,
+                      "us_per_call": us, "TFLOPs": flops / (us * 1e-6) / 1e12,
+                      "mfma_f64_frac": flops / (us * 1e-6) / 78.6e12,
+                      "algorithmic_GBps": nbytes / (us * 1e-6) / 1e9,
+                      "hbm_frac": nbytes / (us * 1e-6) / 8e12}), flush=True)
+
+
+def copy_calibration():
+    """Plain device copy of a known size (256 MiB) for calibrating the
+    FETCH_SIZE / WRITE_SIZE byte counters."""
+    import torch
+    a = torch.ones(64 * 2**20, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": "copy-calibration", "bytes_per_copy": a.numel() * 4}), flush=True)
+
+
 def solve(op, x0, fp="f64"):
     from polfed_amd import DoSConfig, FactorizationConfig, TransformConfig, polfed
     return polfed(op, x0, 100, "middle", produce_report=True,
@@ -144,12 +240,27 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32", action="store_true",
                     help="skip the FP32-filter solves (mixed and pure FP32)")
+    ap.add_argument("--filter-only", type=int, default=0, metavar="STEPS",
+                    help="profiling driver: time about STEPS filter steps only")
+    ap.add_argument("--dtype", default="f64", choices=["f64", "f32"],
+                    help="filter arithmetic for --filter-only")
+    ap.add_argument("--reorth-only", type=int, default=0, metavar="CALLS",
+                    help="profiling driver: time CALLS CGS2 reorthogonalizations")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="profiling driver: one 256 MiB device copy, five times")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "single-GPU build: only --gpus 1 is supported"}))
+        print(json.dumps({"error": "single-GPU build: only --gpus 1 is supported; "
+                                   "the multi-GPU sharding is described, not built"}))
         sys.exit(2)
 
     _load_package()
+    if args.copy_calibration:
+        return copy_calibration()
+    if args.filter_only:
+        return filter_only(args)
+    if args.reorth_only:
+        return reorth_only(args)
     import numpy as np
     import torch
     from polfed_amd.device import Device
@@ -176,6 +287,7 @@ def main():
     ms_step, bytes_step, nsteps, filter_ms = timer.summary(op.D, 8)
     rep = reports[-1]
     achieved = bytes_step / (ms_step * 1e-3) / 1e9
+    kname = kernel_name(op)
     out = {
         "metric": METRIC,
         "value": elapsed / args.steps,
@@ -195,8 +307,8 @@ def main():
                   "eigenconverged": rep.eigenconverged, "max_residual": rep.eigenresidual,
                   "Emin": rep.Emin, "Emax": rep.Emax, "times_s": rep.times},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "chain_step_kernel<double,2>",
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(kname), "kernel": kname,
                      "per_launch_bytes": bytes_step, "avg_launch_ms": ms_step,
                      "filter_share": filter_ms * 1e-3 / elapsed},
     }

@@ -153,7 +153,7 @@ class Operator:
                                    ctypes.byref(vec), ctypes.byref(xcd), self.device.stream))
         return us.value, vec.value, xcd.value
 
-    def ensure_tuned(self, ncols: int, dtype="f64", nsteps: int = 20):
+    def ensure_tuned(self, ncols: int, dtype="f64", nsteps: int = 30):
         """Autotune (dtype, ncols) once per operator."""
         done = self.__dict__.setdefault("_tuned", {})
         if (dtype, ncols) not in done:

@@ -29,6 +29,7 @@ extern "C" {
 typedef struct pf_handle pf_handle;  /* device context + scratch + autotune cache */
 typedef struct pf_op pf_op;          /* matrix-free Hamiltonian on one device     */
 typedef void* pf_stream;             /* hipStream_t                               */
+typedef struct pf_batch pf_batch;    /* many operators filtered in one launch     */
 
 enum {
   PF_OK = 0,
@@ -231,6 +232,54 @@ int pf_residuals(pf_handle* h, pf_op* op, const double* psi, int64_t ld, int32_t
 /* Column dot products out[j] = <A_j, B_j> for j < n (host output), FP64. */
 int pf_coldots(pf_handle* h, const double* A, int64_t lda, const double* B, int64_t ldb,
                int64_t D, int32_t n, double* out_host, pf_stream stream);
+
+/* ------------------------------------------------------------- batching */
+
+/* A batch of n Quantum-Sun operators (disorder realizations, possibly of
+ * different sizes) whose filters run together: each Clenshaw step of every
+ * instance is one multi-instance kernel launch, optionally replayed from a
+ * captured HIP graph.  Every instance filters `ncols` columns (1..32).  The
+ * operators must outlive the batch; their rescaling (pf_op_set_rescale) is
+ * read by pf_batch_set_filter.  Calls on one batch must be ordered (one
+ * stream, or synchronized).  The reference filters one problem per call
+ * (ParallelizationClenshaw.jl:41-51, reached once per Lanczos iteration of
+ * each polfed() call); this is the batched form of that call. */
+int pf_batch_create(pf_handle* h, int32_t n, pf_op* const* ops, int32_t ncols, pf_batch** out);
+int pf_batch_destroy(pf_batch* b);
+
+/* Polynomial of each instance: orders[i] (>= 0) and its orders[i]+1
+ * normalized coefficients, concatenated in instance order in `coeffs`
+ * (host).  Same meaning as pf_filter's coeffs/order. */
+int pf_batch_set_filter(pf_batch* b, int32_t polytype, const int32_t* orders,
+                        const double* coeffs);
+
+/* Y[i] = p_i(H~_i) X[i] for every instance with active[i] != 0, in the
+ * arithmetic dtypes[i] (PF_F64 / PF_F32).  X[i], Y[i]: device FP64
+ * column-major D_i x ncols blocks with leading dimensions ldx[i], ldy[i]
+ * (host arrays of n pointers / values).  Replaces, for a set of problems,
+ * the per-problem whole-block filter call clenshaw(transform, W, V, b, ::GPU,
+ * ::NoParallel) (ParallelizationClenshaw.jl:41-51). */
+int pf_batch_filter(pf_batch* b, const int32_t* active, const int32_t* dtypes,
+                    const double* const* X, const int64_t* ldx, double* const* Y,
+                    const int64_t* ldy, pf_stream stream);
+
+enum {
+  PF_BATCH_GRAPH = 1,          /* 0: stream launches, 1: graph replays (default) */
+  PF_BATCH_GRAPH_STEPS = 2,    /* step nodes per captured graph (default 96)      */
+  PF_BATCH_VEC = 3,            /* columns per lane: 1/2/4/8 dividing ncols        */
+  PF_BATCH_ROWS_PER_LANE = 4,  /* rows per lane: 1/2/4                            */
+  PF_BATCH_XCD = 5             /* 1: XCD-contiguous block mapping (default)       */
+};
+/* Launch settings of the batch.  No reference counterpart. */
+int pf_batch_set_tuning(pf_batch* b, int32_t key, int32_t value);
+
+/* Times every (columns per lane, rows per lane, XCD mapping) tiling of the
+ * multi-instance step kernel on all instances in arithmetic `dtype`, then
+ * graph replays against stream launches, and keeps the fastest.  best_us
+ * (host, may be NULL): time per step of the whole batch.  Overwrites the
+ * batch's work buffers.  No reference counterpart. */
+int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, double* best_us,
+                      pf_stream stream);
 
 #ifdef __cplusplus
 }

@@ -17,3 +17,9 @@ def polfed(*args, **kwargs):
     """See ``polfed_amd.core.polfed``."""
     from .core import polfed as _polfed
     return _polfed(*args, **kwargs)
+
+
+def polfed_batch(*args, **kwargs):
+    """See ``polfed_amd.core.polfed_batch``."""
+    from .core import polfed_batch as _polfed_batch
+    return _polfed_batch(*args, **kwargs)

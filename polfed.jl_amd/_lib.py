@@ -31,6 +31,12 @@ PF_TUNE_VEC = 2
 PF_TUNE_RESET = 3
 PF_TUNE_NONTEMPORAL = 4
 
+PF_BATCH_GRAPH = 1
+PF_BATCH_GRAPH_STEPS = 2
+PF_BATCH_VEC = 3
+PF_BATCH_ROWS_PER_LANE = 4
+PF_BATCH_XCD = 5
+
 PF_PROBE_GAUSSIAN = 0
 PF_PROBE_PHASE = 1
 
@@ -91,6 +97,13 @@ SIGNATURES = {
     "pf_rotate": (_c.c_int, [_vp, _vp, _i64, _i64, _i32, _pdbl, _vp]),
     "pf_residuals": (_c.c_int, [_vp, _vp, _vp, _i64, _i32, _pdbl, _pdbl, _vp]),
     "pf_coldots": (_c.c_int, [_vp, _vp, _i64, _vp, _i64, _i64, _i32, _pdbl, _vp]),
+    "pf_batch_create": (_c.c_int, [_vp, _i32, _c.POINTER(_vp), _i32, _c.POINTER(_vp)]),
+    "pf_batch_destroy": (_c.c_int, [_vp]),
+    "pf_batch_set_filter": (_c.c_int, [_vp, _i32, _pi32, _pdbl]),
+    "pf_batch_filter": (_c.c_int, [_vp, _pi32, _pi32, _c.POINTER(_vp), _c.POINTER(_i64),
+                                   _c.POINTER(_vp), _c.POINTER(_i64), _vp]),
+    "pf_batch_set_tuning": (_c.c_int, [_vp, _i32, _i32]),
+    "pf_batch_autotune": (_c.c_int, [_vp, _i32, _i32, _pdbl, _vp]),
 }
 
 _LIB = None

@@ -157,25 +157,28 @@ def _filter_callable(op: Operator, mapping: MappingConfig, coeffs: np.ndarray, p
     return apply
 
 
-def polfed(op: Operator, x0, howmany: int, target, *, produce_report: bool = False,
-           mapping: Optional[MappingConfig] = None, transform: Optional[TransformConfig] = None,
-           fact: Optional[FactorizationConfig] = None, dos: Optional[DoSConfig] = None,
-           probes: Optional[torch.Tensor] = None, device: Optional[Device] = None):
-    """Eigenpairs of ``op`` closest (by the polynomial filter) to ``target``.
+@dataclass
+class _Prepared:
+    """Per-problem state between the set-up stages and the Lanczos stage."""
+    x0: torch.Tensor
+    s: int
+    Emin: float
+    Emax: float
+    plan: object
+    coeffs: np.ndarray
+    moments: np.ndarray
+    maxdim: int
+    times: dict
 
-    Returns ``(vals, vecs)`` -- vals ascending (host numpy), vecs an
-    (nconv, D) device block -- or ``(vals, vecs, report)`` with
-    ``produce_report=True``.  ``probes`` (D x R device block) overrides the
-    generated KPM probe vectors.
-    """
-    mapping = mapping or MappingConfig()
-    transform = transform or TransformConfig()
-    fact = fact or FactorizationConfig()
-    dos = dos or DoSConfig()
-    dev = device or op.device or default_device()
-    t_all = time.perf_counter()
+
+def _prepare(op: Operator, x0, howmany: int, target, mapping: MappingConfig,
+             transform: TransformConfig, fact: FactorizationConfig, dos: DoSConfig,
+             probes: Optional[torch.Tensor], dev: Device) -> _Prepared:
+    """Everything before the Lanczos stage of polfed_algorithm
+    (PolfedCore.jl:153-263, polfed_algorithm.jl:15-24): initial state, the
+    mapping plan with the Emin/Emax probes, KPM DoS, interval and order, and
+    the normalized filter coefficients."""
     times = {}
-
     x0 = prepare_initial_state(dev, x0)
     s = 1 if x0.dim() == 1 else x0.shape[0]
     D = x0.shape[-1]
@@ -212,18 +215,49 @@ def polfed(op: Operator, x0, howmany: int, target, *, produce_report: bool = Fal
     times["transform"] = time.perf_counter() - t0
     if transform.dtype not in ("f64", "f32", "mixed"):
         raise ValueError(f"TransformConfig.dtype must be 'f64', 'f32' or 'mixed', got {transform.dtype!r}")
-    f_t = _filter_callable(op, mapping, coeffs, plan.polynomialtype, "f64")
+    return _Prepared(x0, s, Emin, Emax, plan, coeffs, moments, maxdim, times)
+
+
+def _report(prep: _Prepared, res: LanczosResult, howmany: int, dtype: str, times: dict) -> Report:
+    plan = prep.plan
+    return Report(prep.Emin, prep.Emax, plan.target, plan.left, plan.right, plan.order, prep.s,
+                  prep.maxdim, res.numiter, res.converged, res.eigenconverged, res.residual,
+                  res.eigenresidual, plan.order * prep.s * res.numiter + howmany, dtype,
+                  prep.moments, times, res)
+
+
+def polfed(op: Operator, x0, howmany: int, target, *, produce_report: bool = False,
+           mapping: Optional[MappingConfig] = None, transform: Optional[TransformConfig] = None,
+           fact: Optional[FactorizationConfig] = None, dos: Optional[DoSConfig] = None,
+           probes: Optional[torch.Tensor] = None, device: Optional[Device] = None):
+    """Eigenpairs of ``op`` closest (by the polynomial filter) to ``target``.
+
+    Returns ``(vals, vecs)`` -- vals ascending (host numpy), vecs an
+    (nconv, D) device block -- or ``(vals, vecs, report)`` with
+    ``produce_report=True``.  ``probes`` (D x R device block) overrides the
+    generated KPM probe vectors.
+    """
+    mapping = mapping or MappingConfig()
+    transform = transform or TransformConfig()
+    fact = fact or FactorizationConfig()
+    dos = dos or DoSConfig()
+    dev = device or op.device or default_device()
+    t_all = time.perf_counter()
+    prep = _prepare(op, x0, howmany, target, mapping, transform, fact, dos, probes, dev)
+    times = dict(prep.times)
+    plan = prep.plan
+    f_t = _filter_callable(op, mapping, prep.coeffs, plan.polynomialtype, "f64")
     f_fast = None
     if transform.dtype != "f64":
-        f_fast = _filter_callable(op, mapping, coeffs, plan.polynomialtype, "f32")
+        f_fast = _filter_callable(op, mapping, prep.coeffs, plan.polynomialtype, "f32")
     fast_mode = {"f64": "never", "f32": "always", "mixed": "mixed"}[transform.dtype]
     # launch autotuning of the step kernel for this block width (cached per operator)
     for dt in (("f64",) if transform.dtype == "f64" else ("f64", "f32")):
-        op.ensure_tuned(min(s, 32 if dt == "f64" else 64), dt)
+        op.ensure_tuned(min(prep.s, 32 if dt == "f64" else 64), dt)
 
     # ---- lanczos on the filtered operator (polfed_algorithm.jl:25-34)
     t0 = time.perf_counter()
-    res = lanczos(dev, f_t, x0, howmany, maxdim=maxdim, which=fact.which, tol=fact.tol,
+    res = lanczos(dev, f_t, prep.x0, howmany, maxdim=prep.maxdim, which=fact.which, tol=fact.tol,
                   eigentol=fact.eigentol if fact.eigentol is not None else 1e-8, mapvals=op,
                   rot=fact.rot, f_fast=f_fast, fast_mode=fast_mode,
                   switch_tol=transform.switch_tol)
@@ -234,9 +268,109 @@ def polfed(op: Operator, x0, howmany: int, target, *, produce_report: bool = Fal
 
     if not produce_report:
         return res.vals, res.vecs
-    iters_needed = res.numiter
-    report = Report(Emin, Emax, plan.target, plan.left, plan.right, plan.order, s, maxdim,
-                    res.numiter, res.converged, res.eigenconverged, res.residual,
-                    res.eigenresidual, plan.order * s * iters_needed + howmany,
-                    transform.dtype, moments, times, res)
-    return res.vals, res.vecs, report
+    return res.vals, res.vecs, _report(prep, res, howmany, transform.dtype, times)
+
+
+@dataclass
+class BatchReport:
+    """Batch-level timing of ``polfed_batch``."""
+    n: int
+    setup_s: float
+    lanczos_s: float
+    filter_s: float
+    total_s: float
+    filter_calls: int
+    batched_steps: int
+    autotune_us: Optional[float]
+    reports: list
+
+
+def polfed_batch(ops, x0s, howmany: int, target, *, produce_report: bool = False,
+                 mapping: Optional[MappingConfig] = None,
+                 transform: Optional[TransformConfig] = None,
+                 fact: Optional[FactorizationConfig] = None, dos: Optional[DoSConfig] = None,
+                 probes=None, device: Optional[Device] = None, autotune: bool = True):
+    """``polfed`` on many Quantum-Sun instances at once (disorder realizations,
+    possibly of different sizes).  Every instance runs the reference algorithm
+    exactly as ``polfed`` does -- its own bounds, DoS, interval, order and
+    block Lanczos -- but the filter applications of all instances that need
+    one are served together by ``QSunBatch`` (one multi-instance kernel launch
+    per Clenshaw step, graph-replayed).  ``transform.dtype`` "f32"/"mixed"
+    applies per instance as in ``polfed``.
+
+    Returns a list of ``(vals, vecs)`` per instance, or
+    ``(list of (vals, vecs, report), BatchReport)`` with ``produce_report``.
+    ``probes``: None or a list of per-instance D_i x R probe blocks.
+    """
+    from .batch import QSunBatch
+    from .lanczos import lanczos_coroutine
+    mapping = mapping or MappingConfig()
+    transform = transform or TransformConfig()
+    fact = fact or FactorizationConfig()
+    dos = dos or DoSConfig()
+    if mapping.clenshaw_recurrence is not None or mapping.clenshaw_finalsum is not None \
+            or mapping.f_rescaled is not None:
+        raise ValueError("polfed_batch: user Clenshaw/mapping hooks are not batched; use polfed")
+    n = len(ops)
+    if len(x0s) != n:
+        raise ValueError("polfed_batch: one initial state per operator")
+    dev = device or ops[0].device or default_device()
+    t_all = time.perf_counter()
+    preps = [_prepare(op, x0, howmany, target, mapping, transform, fact, dos,
+                      None if probes is None else probes[i], dev)
+             for i, (op, x0) in enumerate(zip(ops, x0s))]
+    s = preps[0].s
+    if any(p.s != s for p in preps):
+        raise ValueError("polfed_batch: every instance must use the same block size")
+    polytypes = {p.plan.polynomialtype for p in preps}
+    if len(polytypes) != 1:
+        raise ValueError("polfed_batch: one polynomial family for the whole batch")
+    batch = QSunBatch(ops, s)
+    batch.set_filter([p.coeffs for p in preps], polytypes.pop())
+    tuned = None
+    if autotune:
+        tuned = batch.autotune("f64" if transform.dtype == "f64" else "f32")
+    setup_s = time.perf_counter() - t_all
+
+    fast_mode = {"f64": "never", "f32": "always", "mixed": "mixed"}[transform.dtype]
+    eigentol = fact.eigentol if fact.eigentol is not None else 1e-8
+    gens = [lanczos_coroutine(dev, p.x0, howmany, maxdim=p.maxdim, which=fact.which, tol=fact.tol,
+                              eigentol=eigentol, mapvals=op, rot=fact.rot, stage_timers=False,
+                              has_fast=transform.dtype != "f64", fast_mode=fast_mode,
+                              switch_tol=transform.switch_tol)
+            for op, p in zip(ops, preps)]
+    t0 = time.perf_counter()
+    pending = {i: next(g) for i, g in enumerate(gens)}
+    results = [None] * n
+    events = []
+    calls = 0
+    steps = 0
+    while pending:
+        reqs = [None] * n
+        for i, (X, Y, fast) in pending.items():
+            reqs[i] = (X, Y, "f32" if fast else "f64")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        batch.filter(reqs)
+        e1.record()
+        events.append((e0, e1))
+        calls += 1
+        steps += int(max(batch.orders[i] for i in pending)) + 1
+        for i in list(pending):
+            try:
+                pending[i] = gens[i].send(None)
+            except StopIteration as stop:
+                results[i] = stop.value
+                del pending[i]
+    torch.cuda.synchronize()
+    lanczos_s = time.perf_counter() - t0
+    filter_s = sum(a.elapsed_time(b) for a, b in events) * 1e-3
+    total_s = time.perf_counter() - t_all
+    batch.close()
+    if not produce_report:
+        return [(r.vals, r.vecs) for r in results]
+    out = []
+    for p, r in zip(preps, results):
+        out.append((r.vals, r.vecs, _report(p, r, howmany, transform.dtype, dict(p.times))))
+    return out, BatchReport(n, setup_s, lanczos_s, filter_s, total_s, calls, steps, tuned,
+                            [o[2] for o in out])

@@ -1,0 +1,659 @@
+// Batched multi-instance Clenshaw filter for Quantum-Sun operators.
+//
+// Many independent instances (disorder realizations, uneven sizes) are
+// filtered together: every Clenshaw step of every instance is one launch of
+// qsun_batch_step_kernel over the concatenated rows of all instances.  The
+// instances run in lockstep aligned at the END of their filters: at global
+// step k (Kmax .. 0) instance i is active iff k <= K_i, its local step is
+// g = K_i - k, and its buffer rotation, zero tracking and coefficient index
+// all follow from (k, K_i) on the device.  Instances are laid out in
+// decreasing K, so the active ones at step k are a prefix of the launch
+// order and the stream-mode grid shrinks to them.
+//
+// Graph mode captures M consecutive step nodes, which read the current step
+// from device memory (node j handles k = kstate - j), plus one node that
+// lowers kstate by M; the graph is replayed ceil((Kmax+1)/M) times per
+// filter application.  Nothing in the graph is a memset or a copy.
+//
+// Each instance has its own arithmetic (FP64 or FP32) per application, so a
+// mixed-precision driver can switch instances independently.  Buffers are
+// sized for FP64; an FP32 instance uses the first half of its region.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "pf_device.h"
+#include "pf_internal.h"
+
+namespace pf {
+
+// Static per-instance record (device, launch order).
+struct BatchInst {
+  int32_t Lg, Lloc;
+  int64_t D;
+  int64_t row0;      // first row of this instance in the batch row space
+  int64_t coef_off;  // first filter coefficient of this instance
+  double scale, shift;
+  uint32_t flip[kMaxLoc];
+  double coup[kMaxLoc];
+  double h[kMaxLoc];
+  double G[(1 << kMaxGrain) * (1 << kMaxGrain)];
+};
+
+// Per-application record (device, launch order).
+struct BatchDyn {
+  int32_t K;      // order of this application's filter; -1: not taking part
+  int32_t dtype;  // PF_F64 / PF_F32
+  double* Y;      // FP64 column-major output (device)
+  int64_t ldy;
+};
+
+struct BatchArgs {
+  const BatchInst* insts;
+  const BatchDyn* dyn;
+  const int64_t* blk0;  // n+1 prefix of blocks per instance, launch order
+  const double* coefs;  // null: timing mode (every coefficient 0.5)
+  const int* kstate;    // graph mode: kstate[0] is the step of node 0
+  char* Xb;
+  char* B0;
+  char* B1;
+  char* B2;
+  int n;
+  int polytype;
+  int s;     // columns per instance
+  int G;     // lanes per row
+  int rpb;   // rows per block pass (256 / G)
+  int xcd;   // XCD-contiguous block mapping
+};
+
+__device__ __forceinline__ void poly_ab_dev(int type, int k, double& alpha, double& beta_k) {
+  switch (type) {
+    case PF_LEGENDRE: alpha = (2.0 * k + 1) / (k + 1); beta_k = -(double)k / (k + 1); break;
+    case PF_HERMITE: alpha = 2.0; beta_k = -2.0 * k; break;
+    case PF_TAYLOR: alpha = 1.0; beta_k = 0.0; break;
+    default: alpha = 2.0; beta_k = -1.0; break;  // Chebyshev
+  }
+}
+
+// Buffer slot written at local step g (rotation (3,1,2) of the recurrence).
+__device__ __forceinline__ int rot_slot(int g) { return (3 - g % 3) % 3; }
+
+template <typename T, int V, int RPT>
+__device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& I,
+                                           const BatchDyn& dy, int k, int64_t lb) {
+  typedef typename VT<T, V>::t vt;
+  __shared__ T sG[(1 << kMaxGrain) * (1 << kMaxGrain)];
+  const int dg = 1 << I.Lg;
+  for (int t = threadIdx.x; t < dg * dg; t += blockDim.x) sG[t] = (T)(I.G[t] * I.scale);
+  __syncthreads();
+
+  const int g = dy.K - k;  // local step, >= 0
+  const bool hasH = g >= 1;
+  double cxd, chd, cbd;
+  const double ck = A.coefs ? A.coefs[I.coef_off + k] : 0.5;
+  bool hasB3;
+  if (k >= 1) {
+    double alpha, beta1, dummy;
+    poly_ab_dev(A.polytype, k, alpha, dummy);
+    poly_ab_dev(A.polytype, k + 1, dummy, beta1);
+    cxd = ck;
+    chd = alpha;
+    cbd = beta1;
+  } else {
+    double dummy, beta1;
+    poly_ab_dev(A.polytype, 1, dummy, beta1);
+    const double phi1 = A.polytype == PF_HERMITE ? 2.0 : 1.0;
+    cxd = ck;  // phi0 = 1 for every family
+    chd = phi1;
+    cbd = beta1;
+  }
+  hasB3 = g >= 2 && cbd != 0.0;
+  const T cx = (T)cxd, ch = (T)chd, cb = (T)cbd;
+
+  const int64_t off = I.row0 * (int64_t)A.s * 8;  // byte offset of this instance's region
+  char* bufs[3] = {A.B0, A.B1, A.B2};
+  const T* __restrict__ X = reinterpret_cast<const T*>(A.Xb + off);
+  T* __restrict__ OUT = reinterpret_cast<T*>(bufs[rot_slot(g)] + off);
+  const T* __restrict__ B2 = reinterpret_cast<const T*>(bufs[rot_slot(g + 2)] + off);  // g-1
+  const T* __restrict__ B3 = reinterpret_cast<const T*>(bufs[rot_slot(g + 1)] + off);  // g-2
+
+  const int rloc = threadIdx.x / A.G;
+  const int gl = threadIdx.x - rloc * A.G;
+  if (rloc >= A.rpb) return;
+  const int64_t cofs = (int64_t)gl * V;
+  const uint32_t rs32 = (uint32_t)A.s;
+  const T* __restrict__ B2c = B2 + cofs;
+  const double shift = I.shift, scale = I.scale;
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    const int64_t row = (lb * RPT + u) * A.rpb + rloc;
+    if (row >= I.D) break;
+    vt hv = (vt)(T)0;
+    if (hasH) {
+      const uint32_t r = (uint32_t)row;
+      const int a = (int)(r & (uint32_t)(dg - 1));
+      const uint32_t base = r & ~(uint32_t)(dg - 1);
+      double dd = 0.0;
+      for (int j = 0; j < I.Lloc; ++j) dd += I.h[j] * (((r >> (I.Lg + j)) & 1u) ? 0.5 : -0.5);
+      dd += I.G[a * dg + a];
+      const T d = (T)((dd - shift) * scale);
+      vt acc = (vt)(T)0;
+      for (int b = 0; b < dg; ++b) {
+        const T wb = (b == a) ? d : sG[a * dg + b];
+        acc += wb * vload<T, V>(B2c + (base | (uint32_t)b) * rs32);
+      }
+      for (int j0 = 0; j0 < I.Lloc; j0 += kGatherBatch) {
+        vt x[kGatherBatch];
+        T w[kGatherBatch];
+#pragma unroll
+        for (int q = 0; q < kGatherBatch; ++q) {
+          const int j = j0 + q;
+          const bool ok = j < I.Lloc;
+          w[q] = ok ? (T)(I.coup[j] * scale) : (T)0;
+          x[q] = vload<T, V>(B2c + (ok ? (r ^ I.flip[j]) : r) * rs32);
+        }
+#pragma unroll
+        for (int q = 0; q < kGatherBatch; ++q) acc += w[q] * x[q];
+      }
+      hv = acc;
+    }
+    const int64_t o = row * A.s + cofs;
+    vt out = cx * vload<T, V>(X + o);
+    if (hasH) out += ch * hv;
+    if (hasB3) out += cb * vload<T, V>(B3 + o);
+    if (k >= 1) {
+      vstore<T, V>(OUT + o, out);
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        dy.Y[row + (int64_t)(gl * V + v) * dy.ldy] = (double)vget<T, V>(out, v);
+    }
+  }
+}
+
+// One Clenshaw step of every active instance.  kexpl >= 0: the step index;
+// kexpl < 0: k = kstate[0] - j (graph nodes).
+template <int V, int RPT>
+__global__ __launch_bounds__(256) void qsun_batch_step_kernel(const BatchArgs A, int kexpl,
+                                                              int j) {
+  const int64_t nb = gridDim.x;
+  int64_t b = blockIdx.x;
+  if (A.xcd) b = xcd_block(b, nb);
+  int lo = 0, hi = A.n;  // blk0[lo] <= b < blk0[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (A.blk0[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const BatchDyn dy = A.dyn[lo];
+  const int k = kexpl >= 0 ? kexpl : A.kstate[0] - j;
+  if (k < 0 || dy.K < 0 || k > dy.K) return;  // uniform over the block
+  const BatchInst& I = A.insts[lo];
+  const int64_t lb = b - A.blk0[lo];
+  if (dy.dtype == PF_F64) {
+    batch_body<double, V, RPT>(A, I, dy, k, lb);
+  } else {
+    batch_body<float, V, RPT>(A, I, dy, k, lb);
+  }
+}
+
+__global__ void batch_kset_kernel(int* kstate, int value) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) kstate[0] = value;
+}
+__global__ void batch_kdec_kernel(int* kstate, int m) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) kstate[0] -= m;
+}
+
+static bool batch_vec_ok(int s, int V) {
+  return (V == 1 || V == 2 || V == 4 || V == 8) && s % V == 0 && s / V <= 256;
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+struct pf_batch {
+  pf_handle* h = nullptr;
+  int n = 0, s = 1;
+  std::vector<pf_op*> ops;
+  std::vector<int> order;      // launch position -> instance
+  std::vector<int> pos;        // instance -> launch position
+  std::vector<int64_t> row0;   // launch order
+  std::vector<int64_t> blk0;   // launch order, n+1
+  std::vector<int32_t> K;      // launch order (-1 before a filter is set)
+  std::vector<int64_t> coef_off;  // launch order
+  int64_t rows_total = 0;
+  int polytype = PF_CHEBYSHEV;
+  bool have_filter = false;
+  DevBuf insts, dyn, blk0d, coefs, kstate, bufs;
+  // tuning
+  int V = 1, rpt = 1, xcd = 1, use_graph = 1, gsteps = 96;
+  // pinned staging of the per-application records; the event marks the end
+  // of the last copy out of it
+  BatchDyn* hdyn = nullptr;
+  hipEvent_t dyn_done = nullptr;
+  bool dyn_pending = false;
+  // graph state
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  int G() const { return s / V; }
+  int rpb() const { return 256 / G(); }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  // Stage per-application records and copy them to the device on st.
+  void upload_dyn(const std::vector<BatchDyn>& dy, hipStream_t st) {
+    if (!hdyn) {
+      PF_HIP(hipHostMalloc(reinterpret_cast<void**>(&hdyn), n * sizeof(BatchDyn)));
+      PF_HIP(hipEventCreateWithFlags(&dyn_done, hipEventDisableTiming));
+    }
+    if (dyn_pending) PF_HIP(hipEventSynchronize(dyn_done));
+    std::memcpy(hdyn, dy.data(), n * sizeof(BatchDyn));
+    PF_HIP(hipMemcpyAsync(dyn.p, hdyn, n * sizeof(BatchDyn), hipMemcpyHostToDevice, st));
+    PF_HIP(hipEventRecord(dyn_done, st));
+    dyn_pending = true;
+  }
+  // block prefix in launch order for the current tiling (after the device is
+  // idle: kernels of an earlier call may still read the old table)
+  void layout_blocks() {
+    PF_HIP(hipDeviceSynchronize());
+    blk0.assign(n + 1, 0);
+    const int64_t per = (int64_t)rpb() * rpt;
+    for (int p = 0; p < n; ++p) {
+      const int64_t D = ops[order[p]]->D;
+      blk0[p + 1] = blk0[p] + (D + per - 1) / per;
+    }
+    PF_REQUIRE(blk0[n] < ((int64_t)1 << 31), "batch: grid too large");
+    int64_t* d = reinterpret_cast<int64_t*>(blk0d.get((n + 1) * sizeof(int64_t)));
+    PF_HIP(hipMemcpy(d, blk0.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    drop_graph();
+  }
+  // instance records in launch order
+  void upload_insts() {
+    PF_HIP(hipDeviceSynchronize());
+    std::vector<BatchInst> recs(n);
+    int64_t r0 = 0;
+    row0.assign(n, 0);
+    for (int p = 0; p < n; ++p) {
+      const pf_op* op = ops[order[p]];
+      BatchInst& I = recs[p];
+      std::memset(&I, 0, sizeof(I));
+      I.Lg = op->qsun.Lg;
+      I.Lloc = op->qsun.Lloc;
+      I.D = op->D;
+      I.row0 = r0;
+      I.coef_off = coef_off.empty() ? 0 : coef_off[p];
+      const Affine af = op->affine(true);
+      I.scale = af.scale;
+      I.shift = af.shift;
+      for (int j = 0; j < kMaxLoc; ++j) {
+        I.flip[j] = op->qsun.flip[j];
+        I.coup[j] = op->qsun.coup[j];
+        I.h[j] = op->qsun.h[j];
+      }
+      std::memcpy(I.G, op->qsun.G, sizeof(I.G));
+      row0[p] = r0;
+      r0 += op->D;
+    }
+    rows_total = r0;
+    void* d = insts.get(n * sizeof(BatchInst));
+    PF_HIP(hipMemcpy(d, recs.data(), n * sizeof(BatchInst), hipMemcpyHostToDevice));
+  }
+  BatchArgs args() {
+    BatchArgs A{};
+    A.insts = reinterpret_cast<const BatchInst*>(insts.p);
+    A.dyn = reinterpret_cast<const BatchDyn*>(dyn.p);
+    A.blk0 = reinterpret_cast<const int64_t*>(blk0d.p);
+    A.coefs = reinterpret_cast<const double*>(coefs.p);
+    A.kstate = reinterpret_cast<const int*>(kstate.p);
+    char* base = reinterpret_cast<char*>(bufs.p);
+    const size_t reg = (size_t)rows_total * s * 8;
+    A.Xb = base;
+    A.B0 = base + reg;
+    A.B1 = base + 2 * reg;
+    A.B2 = base + 3 * reg;
+    A.n = n;
+    A.polytype = polytype;
+    A.s = s;
+    A.G = G();
+    A.rpb = rpb();
+    A.xcd = xcd;
+    return A;
+  }
+  void launch(const BatchArgs& A, int64_t grid, int kexpl, int j, hipStream_t st) {
+    if (grid <= 0) return;
+#define PF_BATCH_LAUNCH(VV, RR)                                                               \
+  hipLaunchKernelGGL((qsun_batch_step_kernel<VV, RR>), dim3((unsigned)grid), dim3(256), 0, st, \
+                     A, kexpl, j)
+#define PF_BATCH_RPT(VV)                       \
+  switch (rpt) {                               \
+    case 2: PF_BATCH_LAUNCH(VV, 2); break;     \
+    case 4: PF_BATCH_LAUNCH(VV, 4); break;     \
+    default: PF_BATCH_LAUNCH(VV, 1); break;    \
+  }
+    switch (V) {
+      case 2: PF_BATCH_RPT(2); break;
+      case 4: PF_BATCH_RPT(4); break;
+      case 8: PF_BATCH_RPT(8); break;
+      default: PF_BATCH_RPT(1); break;
+    }
+#undef PF_BATCH_RPT
+#undef PF_BATCH_LAUNCH
+    PF_HIP(hipGetLastError());
+  }
+  // timing: capture with constant coefficients (no coefficient reads), for
+  // the autotuner's out-of-range step indices
+  void ensure_graph(bool timing = false) {
+    if (exec) return;
+    if (!cap) PF_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    BatchArgs A = args();
+    if (timing) A.coefs = nullptr;
+    PF_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int j = 0; j < gsteps; ++j) launch(A, blk0[n], -1, j, cap);
+      hipLaunchKernelGGL(batch_kdec_kernel, dim3(1), dim3(64), 0, cap,
+                         reinterpret_cast<int*>(kstate.p), gsteps);
+      PF_HIP(hipGetLastError());
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(cap, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    PF_HIP(hipStreamEndCapture(cap, &graph));
+    PF_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  // Run steps kmax .. 0 of the current application on stream st.
+  void run_steps(int kmax, hipStream_t st) {
+    const BatchArgs A = args();
+    if (use_graph) {
+      ensure_graph();
+      hipLaunchKernelGGL(batch_kset_kernel, dim3(1), dim3(64), 0, st,
+                         reinterpret_cast<int*>(kstate.p), kmax);
+      PF_HIP(hipGetLastError());
+      const int reps = (kmax + 1 + gsteps - 1) / gsteps;
+      for (int r = 0; r < reps; ++r) PF_HIP(hipGraphLaunch(exec, st));
+    } else {
+      // active at step k: launch positions with K >= k, a prefix that grows
+      // as k falls (K is non-increasing in launch order)
+      int nact = 0;
+      for (int k = kmax; k >= 0; --k) {
+        while (nact < n && K[nact] >= k) ++nact;
+        launch(A, blk0[nact], k, 0, st);
+      }
+    }
+  }
+  ~pf_batch() {
+    (void)hipDeviceSynchronize();
+    drop_graph();
+    if (cap) (void)hipStreamDestroy(cap);
+    if (hdyn) (void)hipHostFree(hdyn);
+    if (dyn_done) (void)hipEventDestroy(dyn_done);
+    insts.release();
+    dyn.release();
+    blk0d.release();
+    coefs.release();
+    kstate.release();
+    bufs.release();
+  }
+};
+
+extern "C" int pf_batch_create(pf_handle* h, int32_t n, pf_op* const* ops, int32_t ncols,
+                               pf_batch** out) {
+  return guarded([&] {
+    PF_REQUIRE(h && ops && out, "batch: null argument");
+    PF_REQUIRE(n >= 1 && n <= 4096, "batch: 1 <= n <= 4096");
+    PF_REQUIRE(ncols >= 1 && ncols <= 32, "batch: 1 <= ncols <= 32");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+      PF_REQUIRE(ops[i], "batch: null operator");
+      PF_REQUIRE(ops[i]->kind == kQsun, "batch: only Quantum-Sun operators can be batched");
+      PF_REQUIRE(ops[i]->h == h, "batch: operators must belong to this handle");
+    }
+    PF_HIP(hipSetDevice(h->device));
+    pf_batch* b = new pf_batch();
+    try {
+      b->h = h;
+      b->n = n;
+      b->s = ncols;
+      b->ops.assign(ops, ops + n);
+      b->order.resize(n);
+      std::iota(b->order.begin(), b->order.end(), 0);
+      b->pos = b->order;
+      b->K.assign(n, -1);
+      b->V = ncols % 2 == 0 ? 2 : 1;
+      b->upload_insts();
+      b->layout_blocks();
+      b->kstate.get(256);
+      b->dyn.get(n * sizeof(BatchDyn));
+      b->bufs.get(4 * (size_t)b->rows_total * ncols * 8);
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+extern "C" int pf_batch_destroy(pf_batch* b) {
+  return guarded([&] { delete b; });
+}
+
+extern "C" int pf_batch_set_filter(pf_batch* b, int32_t polytype, const int32_t* orders,
+                                   const double* coeffs) {
+  return guarded([&] {
+    PF_REQUIRE(b && orders && coeffs, "batch: null argument");
+    PF_REQUIRE(polytype >= PF_CHEBYSHEV && polytype <= PF_TAYLOR, "batch: polynomial type");
+    const int n = b->n;
+    std::vector<int64_t> off_inst(n);
+    int64_t tot = 0;
+    for (int i = 0; i < n; ++i) {
+      PF_REQUIRE(orders[i] >= 0 && orders[i] < (1 << 30), "batch: 0 <= order < 2^30");
+      off_inst[i] = tot;
+      tot += orders[i] + 1;
+    }
+    // launch order: decreasing order, ties by instance index
+    std::vector<int> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return orders[x] > orders[y]; });
+    b->order = ord;
+    for (int p = 0; p < n; ++p) b->pos[ord[p]] = p;
+    b->K.resize(n);
+    b->coef_off.resize(n);
+    for (int p = 0; p < n; ++p) {
+      b->K[p] = orders[ord[p]];
+      b->coef_off[p] = off_inst[ord[p]];
+    }
+    b->polytype = polytype;
+    void* d = b->coefs.get(tot * sizeof(double));
+    PF_HIP(hipMemcpy(d, coeffs, tot * sizeof(double), hipMemcpyHostToDevice));
+    b->upload_insts();   // row0 and coefficient offsets follow the launch order
+    b->layout_blocks();  // drops the graph
+    b->have_filter = true;
+  });
+}
+
+extern "C" int pf_batch_filter(pf_batch* b, const int32_t* active, const int32_t* dtypes,
+                               const double* const* X, const int64_t* ldx, double* const* Y,
+                               const int64_t* ldy, pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(b && active && dtypes && X && ldx && Y && ldy, "batch: null argument");
+    PF_REQUIRE(b->have_filter, "batch: pf_batch_set_filter first");
+    hipStream_t st = S(stream);
+    const int n = b->n;
+    std::vector<BatchDyn> dy(n);
+    int kmax = -1;
+    for (int i = 0; i < n; ++i) {
+      const int p = b->pos[i];
+      BatchDyn& d = dy[p];
+      d.K = -1;
+      d.dtype = PF_F64;
+      d.Y = nullptr;
+      d.ldy = 0;
+      if (!active[i]) continue;
+      const int64_t D = b->ops[i]->D;
+      PF_REQUIRE(X[i] && Y[i], "batch: null block of an active instance");
+      PF_REQUIRE(ldx[i] >= D && ldy[i] >= D, "batch: leading dimension < D");
+      PF_REQUIRE(dtypes[i] == PF_F64 || dtypes[i] == PF_F32, "batch: dtype");
+      d.K = b->K[p];
+      d.dtype = dtypes[i];
+      d.Y = Y[i];
+      d.ldy = ldy[i];
+      kmax = std::max(kmax, d.K);
+    }
+    if (kmax < 0) return;
+    b->upload_dyn(dy, st);
+    char* base = reinterpret_cast<char*>(b->bufs.p);
+    for (int i = 0; i < n; ++i) {
+      if (!active[i]) continue;
+      const int p = b->pos[i];
+      pack_block(dtypes[i], X[i], ldx[i], base + b->row0[p] * (int64_t)b->s * 8, b->ops[i]->D,
+                 b->s, nullptr, st);
+    }
+    b->run_steps(kmax, st);
+  });
+}
+
+extern "C" int pf_batch_set_tuning(pf_batch* b, int32_t key, int32_t value) {
+  return guarded([&] {
+    PF_REQUIRE(b, "batch: null argument");
+    switch (key) {
+      case PF_BATCH_GRAPH:
+        PF_REQUIRE(value == 0 || value == 1, "batch: graph 0/1");
+        b->use_graph = value;
+        break;
+      case PF_BATCH_GRAPH_STEPS:
+        PF_REQUIRE(value >= 1 && value <= 4096, "batch: 1 <= graph steps <= 4096");
+        b->gsteps = value;
+        b->drop_graph();
+        break;
+      case PF_BATCH_VEC:
+        PF_REQUIRE(batch_vec_ok(b->s, value), "batch: vector width must divide ncols (1/2/4/8)");
+        b->V = value;
+        b->layout_blocks();
+        break;
+      case PF_BATCH_ROWS_PER_LANE:
+        PF_REQUIRE(value == 1 || value == 2 || value == 4, "batch: rows per lane 1/2/4");
+        b->rpt = value;
+        b->layout_blocks();
+        break;
+      case PF_BATCH_XCD:
+        PF_REQUIRE(value == 0 || value == 1, "batch: xcd 0/1");
+        b->xcd = value;
+        b->drop_graph();
+        break;
+      default: throw Error{PF_ERR_INVALID, "batch: unknown tuning key"};
+    }
+  });
+}
+
+extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, double* best_us,
+                                 pf_stream stream) {
+  return guarded([&] {
+    PF_REQUIRE(b, "batch: null argument");
+    PF_REQUIRE(dtype == PF_F64 || dtype == PF_F32, "batch: dtype");
+    PF_REQUIRE(nsteps >= 3 && nsteps <= 100000, "batch: 3 <= nsteps <= 100000");
+    hipStream_t st = S(stream);
+    const int n = b->n;
+    // timing mode: every instance active at every step, constant coefficients
+    std::vector<BatchDyn> dy(n);
+    for (auto& d : dy) {
+      d.K = 1 << 29;
+      d.dtype = dtype;
+      d.Y = nullptr;
+      d.ldy = 0;
+    }
+    b->upload_dyn(dy, st);
+    PF_HIP(hipMemsetAsync(b->bufs.p, 0, b->bufs.bytes, st));
+    hipEvent_t e0, e1;
+    PF_HIP(hipEventCreate(&e0));
+    PF_HIP(hipEventCreate(&e1));
+    struct Cand {
+      int V, rpt, xcd;
+      double us;
+    };
+    std::vector<Cand> cands;
+    for (int V : {1, 2, 4, 8})
+      if (batch_vec_ok(b->s, V) && !(dtype == PF_F64 && V > 8))
+        for (int rpt : {1, 2, 4})
+          for (int x : {1, 0}) cands.push_back({V, rpt, x, 1e300});
+    const int saveV = b->V, saveR = b->rpt, saveX = b->xcd;
+    const bool had_filter = b->have_filter;
+    for (int round = 0; round < 3; ++round) {
+      for (auto& c : cands) {
+        b->V = c.V;
+        b->rpt = c.rpt;
+        b->xcd = c.xcd;
+        b->layout_blocks();
+        BatchArgs A = b->args();
+        A.coefs = nullptr;
+        const int64_t grid = b->blk0[n];
+        const int kbig = 1 << 28;
+        for (int w = 0; w < 2; ++w) b->launch(A, grid, kbig, 0, st);
+        PF_HIP(hipEventRecord(e0, st));
+        for (int k = 0; k < nsteps; ++k) b->launch(A, grid, kbig - k, 0, st);
+        PF_HIP(hipEventRecord(e1, st));
+        PF_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        c.us = std::min(c.us, 1e3 * ms / nsteps);
+      }
+    }
+    double best = 1e300;
+    Cand bc{saveV, saveR, saveX, 1e300};
+    for (const auto& c : cands)
+      if (c.us < 0.99 * best) {
+        best = c.us;
+        bc = c;
+      }
+    b->V = bc.V;
+    b->rpt = bc.rpt;
+    b->xcd = bc.xcd;
+    b->layout_blocks();
+    // graph replays against stream launches for the chosen tiling
+    {
+      const int saveG = b->gsteps;
+      b->gsteps = std::min(saveG, nsteps);
+      b->drop_graph();
+      b->ensure_graph(true);
+      double g_us = 1e300, s_us = 1e300;
+      for (int round = 0; round < 3; ++round) {
+        hipLaunchKernelGGL(batch_kset_kernel, dim3(1), dim3(64), 0, st,
+                           reinterpret_cast<int*>(b->kstate.p), 1 << 28);
+        PF_HIP(hipGetLastError());
+        PF_HIP(hipGraphLaunch(b->exec, st));
+        PF_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < 2; ++r) PF_HIP(hipGraphLaunch(b->exec, st));
+        PF_HIP(hipEventRecord(e1, st));
+        PF_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        g_us = std::min(g_us, 1e3 * ms / (2 * b->gsteps));
+        BatchArgs A = b->args();
+        A.coefs = nullptr;
+        PF_HIP(hipEventRecord(e0, st));
+        for (int k = 0; k < 2 * b->gsteps; ++k) b->launch(A, b->blk0[n], (1 << 27) - k, 0, st);
+        PF_HIP(hipEventRecord(e1, st));
+        PF_HIP(hipEventSynchronize(e1));
+        PF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        s_us = std::min(s_us, 1e3 * ms / (2 * b->gsteps));
+      }
+      b->gsteps = saveG;
+      b->drop_graph();
+      b->use_graph = g_us < s_us ? 1 : 0;
+      best = std::min(g_us, s_us);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->have_filter = had_filter;
+    if (best_us) *best_us = best;
+  });
+}

@@ -131,6 +131,46 @@ def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | Non
     early ones may not), and returns to ``f`` if the final residuals miss
     ``eigentol``.
     """
+    gen = lanczos_coroutine(dev, x0, howmany, maxdim=maxdim, which=which, tol=tol,
+                            eigentol=eigentol, mapvals=mapvals, rot=rot,
+                            stage_timers=stage_timers, has_fast=f_fast is not None,
+                            fast_mode=fast_mode, switch_tol=switch_tol)
+    try:
+        X, Y, fast = next(gen)
+        while True:
+            (f_fast if fast else f)(X, Y)
+            X, Y, fast = gen.send(None)
+    except StopIteration as stop:
+        return stop.value
+
+
+def _proj_eigh(T: np.ndarray, s: int, subset=None):
+    """Eigenpairs of the block-tridiagonal projected matrix T (upper triangle
+    valid, bandwidth s: the off-diagonal blocks are upper-triangular R
+    factors) by LAPACK's banded symmetric solver; subset = [lo, hi] selects
+    eigenvalues by index (ascending)."""
+    n = T.shape[0]
+    w = min(s, n - 1)
+    band = np.zeros((w + 1, n))
+    for d in range(w + 1):
+        band[w - d, d:] = np.diagonal(T, offset=d)
+    if subset is None:
+        return sla.eig_banded(band, lower=False, check_finite=False)
+    return sla.eig_banded(band, lower=False, check_finite=False, select="i",
+                          select_range=(subset[0], subset[1]))
+
+
+def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
+                      maxdim: int | None = None, which: str = "SR", tol: float = 1e-14,
+                      eigentol: float = 1e-8, mapvals=None, rot=None, stage_timers: bool = True,
+                      has_fast: bool = False, fast_mode: str = "never",
+                      switch_tol: float = 1e-6):
+    """The block Lanczos of ``lanczos`` as a generator: it yields every
+    operator application it needs as ``(X, Y, fast)`` -- write f(X) (or
+    f_fast(X) when ``fast``) into Y, then resume it with ``send(None)`` -- and
+    returns the ``LanczosResult``.  A batch driver can so advance many
+    independent solves in lockstep and serve their requests with one
+    multi-instance filter launch per step."""
     rot = rot if rot is not None else FullRO()
     if maxdim is None:
         maxdim = 500 * howmany
@@ -153,7 +193,7 @@ def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | Non
     basis[0:s].copy_(X0)
     k = s
     with tm("mapping"):
-        f(basis[0:s], r)
+        yield basis[0:s], r, False
     a = sym_upper(overlap(basis[0:s], r))
     mat[0:s, 0:s] = a
     bo.update(dev, basis[0:s], _to_dev_matrix(a, r), r, alpha=-1.0, beta=1.0)
@@ -166,7 +206,7 @@ def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | Non
     vecs = torch.empty((0, D), dtype=torch.float64, device=X0.device)
     res_out = np.zeros(0)
 
-    use_fast = fast_mode == "always" and f_fast is not None
+    use_fast = fast_mode == "always" and has_fast
     fast_iters = 0
     switch_it = -1
     fallback = False
@@ -183,7 +223,7 @@ def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | Non
             mat[k:k + s, k - s:k] = R
             mat[k - s:k, k:k + s] = R.T
             k += s
-        if fast_mode == "mixed" and f_fast is not None and not use_fast and not fallback \
+        if fast_mode == "mixed" and has_fast and not use_fast and not fallback \
                 and k - s >= howmany:
             with tm("switch_test"):
                 kp = k - s
@@ -193,8 +233,7 @@ def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | Non
                     sub = [0, howmany - 1]
                 else:
                     sub = None
-                lam_p, ph_p = sla.eigh(mat[:kp, :kp], lower=False, check_finite=False,
-                                       subset_by_index=sub)
+                lam_p, ph_p = _proj_eigh(mat[:kp, :kp], s, sub)
                 if sub is None:
                     ph_p = ph_p[:, sortvals(lam_p, which)[:howmany]]
                 est = float(np.max(np.linalg.norm(R @ ph_p[kp - s:kp, :], axis=0)))
@@ -204,11 +243,9 @@ def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | Non
         V_last, V_prev = basis[k - s:k], basis[k - 2 * s:k - s]
         # [2] mapping!
         with tm("mapping"):
+            yield V_last, r, use_fast
             if use_fast:
-                f_fast(V_last, r)
                 fast_iters += 1
-            else:
-                f(V_last, r)
         # [3] calculate_overlap!  [4] orthogonalization!
         with tm("orthogonalization"):
             a = sym_upper(overlap(V_last, r))
@@ -230,7 +267,7 @@ def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | Non
         if nextcheck == numiter or numiter >= maxiter:
             with tm("convergence"):
                 T = mat[:k, :k]
-                lam, phi = sla.eigh(T, lower=False, check_finite=False)
+                lam, phi = _proj_eigh(T, s)
                 idxs = sortvals(lam, which)
                 bnorm = mat[k - s:k, k - 2 * s:k - s]
                 converged = 0

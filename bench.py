@@ -177,6 +177,206 @@ def reorth_only(args):
                       "hbm_frac": nbytes / (us * 1e-6) / 8e12}), flush=True)
 
 
+def qsun_batch_instances(dev, n=64):
+    """64 Quantum-Sun disorder realizations: L_grain=3 (8x8 GOE grain scaled
+    by 1/sqrt(9)), L_loc = 10..14 round-robin (D = 8192..131072), alpha=0.75,
+    zeta=0.2, g0=1, fields U[0.5,1.5]; numpy seeds 1000+i for the draws and
+    2000+i for the start vectors."""
+    import numpy as np
+    from polfed_amd import models
+    ops, x0s = [], []
+    for i in range(n):
+        L_loc = 10 + i % 5
+        op = models.qsun_hamiltonian(L_loc, 3, 1.0, 0.75, rng=np.random.default_rng(1000 + i),
+                                     device=dev)
+        ops.append(op)
+        x0s.append(np.random.default_rng(2000 + i).standard_normal(op.D))
+    return ops, x0s
+
+
+def qsun_batch(args):
+    """Batched workload: nev=32 pairs of each of 64 Quantum-Sun instances at
+    the middle of the spectrum (single-vector Lanczos, eigentol 1e-10).
+    Timed: the batched mixed-precision solve (FP32 filters once a pair set
+    is nearly converged), the batched FP64 solve, and with --plain the plain
+    FP64 path (one polfed call per instance)."""
+    import numpy as np
+    import torch
+    from polfed_amd import FactorizationConfig, TransformConfig, polfed, polfed_batch
+    from polfed_amd.device import Device
+    dev = Device(0)
+    ops, x0s = qsun_batch_instances(dev, args.instances)
+    fact = FactorizationConfig(eigentol=1e-10)
+    out = {"metric": METRIC, "workload": f"{len(ops)} Quantum-Sun instances, L_grain=3, "
+           "L_loc=10..14 round-robin (D=8192..131072), nev=32 each, target middle, "
+           "eigentol 1e-10", "unit": "s", "higher_is_better": False}
+
+    def summarize(res_list, t):
+        conv = [int(len(v)) for v, _, _ in res_list]
+        maxres = max(float(r.lanczos.all_residuals.max()) if r.lanczos.all_residuals is not None
+                     and r.lanczos.all_residuals.size else 0.0 for _, _, r in res_list)
+        return {"time_s": t, "pairs_total": int(sum(conv)), "instances_complete":
+                int(sum(c == 32 for c in conv)), "max_residual": maxres,
+                "iterations_max": int(max(r.iterations for _, _, r in res_list)),
+                "orders": [int(r.order) for _, _, r in res_list]}
+
+    results = {}
+    for mode in ("mixed", "f64"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res, brep = polfed_batch(ops, x0s, 32, "middle", produce_report=True, fact=fact,
+                                 transform=TransformConfig(dtype=mode), progress=log)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        results[mode] = res
+        e = summarize(res, t)
+        e.update({"setup_s": brep.setup_s, "lanczos_s": brep.lanczos_s,
+                  "filter_s": brep.filter_s, "batched_steps": brep.batched_steps,
+                  "filter_calls": brep.filter_calls, "autotune_us_per_step": brep.autotune_us,
+                  "fp32_iterations": int(sum(r.lanczos.fast_iterations for _, _, r in res))})
+        e.pop("orders")
+        out[f"batched_{mode}"] = e
+        log(f"batched {mode}: {t:.1f} s pairs={e['pairs_total']} maxres={e['max_residual']:.2e}")
+    ref = results["f64"]
+    errs = []
+    for (v1, _, r1), (v2, _, r2) in zip(results["mixed"], ref):
+        n = min(len(v1), len(v2))
+        if n:
+            errs.append(float(np.max(np.abs(v1[:n] - v2[:n])) / (r2.Emax - r2.Emin)))
+    out["batched_mixed"]["max_eig_err_vs_fp64_over_width"] = max(errs) if errs else None
+    out["batched_mixed"]["speedup_vs_batched_fp64"] = (out["batched_f64"]["time_s"]
+                                                       / out["batched_mixed"]["time_s"])
+    if args.plain:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plain = []
+        for i, (op, x0) in enumerate(zip(ops, x0s)):
+            plain.append(polfed(op, x0, 32, "middle", produce_report=True, fact=fact))
+            log(f"plain f64 instance {i + 1}/{len(ops)}: D={op.D} "
+                f"{time.perf_counter() - t0:.1f} s")
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        e = summarize(plain, t)
+        e.pop("orders")
+        out["plain_f64"] = e
+        out["batched_mixed"]["speedup_vs_plain_fp64"] = t / out["batched_mixed"]["time_s"]
+        log(f"plain f64: {t:.1f} s pairs={e['pairs_total']}")
+    print(json.dumps(out), flush=True)
+
+
+def xxz_l22(args):
+    """Precision workload: XXZ L=22 Sz=0 (D=705432), W=2 (numpy seed 2),
+    block 16, nev=200, target middle, eigentol 1e-10.  The FP64-filter solve,
+    then the mixed and pure-FP32 filter solves from the same start block (same
+    probes, so the same order K); every eigenvalue and residual is reported
+    against the FP64 run."""
+    import numpy as np
+    import torch
+    from polfed_amd import FactorizationConfig, TransformConfig, models, polfed
+    from polfed_amd.device import Device
+    dev = Device(0)
+    op = models.xxz_hamiltonian(22, 11, 1.0, 1.0, 2.0, rng=np.random.default_rng(2), device=dev)
+    q, _ = np.linalg.qr(np.random.default_rng(102).standard_normal((op.D, 16)))
+    x0 = dev.to_device(q)
+    out = {"metric": METRIC, "workload": "XXZ L=22 Sz=0 D=705432 W=2 seed 2, block 16, "
+           "nev=200, target middle, eigentol 1e-10", "unit": "s", "higher_is_better": False}
+    ref = None
+    for mode in ("f64", "mixed", "f32"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        vals, vecs, rep = polfed(op, x0, 200, "middle", produce_report=True,
+                                 transform=TransformConfig(dtype=mode),
+                                 fact=FactorizationConfig(eigentol=1e-10))
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        lz = rep.lanczos
+        allv = np.sort(lz.all_vals) if lz.all_vals is not None else np.zeros(0)
+        allr = lz.all_residuals if lz.all_residuals is not None else np.zeros(0)
+        e = {"time_s": t, "order": rep.order, "iterations": rep.iterations,
+             "returned_pairs": int(len(vals)), "lanczos_converged": int(lz.converged),
+             "max_residual": float(allr.max()) if allr.size else None,
+             "median_residual": float(np.median(allr)) if allr.size else None,
+             "fp32_iterations": int(lz.fast_iterations), "fp64_fallback": bool(lz.fallback)}
+        if ref is None:
+            ref = (allv, rep.Emax - rep.Emin, t)
+        else:
+            n = min(len(allv), len(ref[0]))
+            err = np.abs(allv[:n] - ref[0][:n]) / ref[1]
+            e.update({"speedup_vs_fp64": ref[2] / t,
+                      "eig_err_over_width_max": float(err.max()) if n else None,
+                      "eig_err_over_width_median": float(np.median(err)) if n else None})
+        out[mode] = e
+        log(f"l22 {mode}: {t:.1f} s K={rep.order} iters={rep.iterations} pairs={len(vals)} "
+            f"maxres={e['max_residual']}")
+    print(json.dumps(out), flush=True)
+
+
+def j1j2_l24(args):
+    """J1-J2 L=24 Sz=0 (D=2704156), J1=1, J2=0.5: KPM with 32 random-phase
+    probes and 2048 Chebyshev moments (Jackson), FP32 against FP64 on the same
+    probes; then the DoS-chosen target (maximal density), the order for nev=300
+    and one filter application on a 32-column block in FP32 and FP64, timed
+    (the whole solve, 46 iterations of such filters, is projected from it)."""
+    import numpy as np
+    import torch
+    from polfed_amd import TransformConfig, models
+    from polfed_amd import blockops as bo
+    from polfed_amd import spectral as spc
+    from polfed_amd.configs import PolfedDefaults
+    from polfed_amd.device import Device
+    dev = Device(0)
+    op = models.j1j2_hamiltonian(24, 12, 1.0, 0.5, 1.0, 1.0, 0.0, device=dev)
+    # spectral bounds from the reference's Lanczos probe (unscaled operator)
+    from polfed_amd.core import _extremal
+    v0 = dev.to_device(np.random.default_rng(5).standard_normal(op.D))
+    v0 = v0 / torch.linalg.vector_norm(v0)
+    t0 = time.perf_counter()
+    Emin, Emax = _extremal(dev, op, v0, "SR"), _extremal(dev, op, v0, "LR")
+    t_bounds = time.perf_counter() - t0
+    op.set_rescale(Emin, Emax)
+    P = bo.fill_probes(dev, op.D, 32, seed=24, kind="phase")
+    out = {"metric": METRIC, "workload": "J1-J2 L=24 Sz=0 D=2704156 J1=1 J2=0.5; KPM R=32 "
+           "random-phase probes, N=2048 Jackson; nev=300 at the maximal-density target, block 32",
+           "Emin": Emin, "Emax": Emax, "bounds_s": t_bounds}
+    mus = {}
+    for dt in ("f64", "f32"):
+        op.ensure_tuned(32, dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mus[dt] = op.kpm_moments(2048, P, dtype=dt)
+        torch.cuda.synchronize()
+        out[f"kpm_{dt}_s"] = time.perf_counter() - t0
+        log(f"kpm {dt}: {out[f'kpm_{dt}_s']:.2f} s")
+    d = np.abs(mus["f32"] - mus["f64"]) / op.D
+    out["kpm_moment_err_max_over_D"] = float(d.max())
+    out["kpm_moment_err_median_over_D"] = float(np.median(d))
+    a, b = (Emax - Emin) / 2, (Emax + Emin) / 2
+    rho = spc.DensityOfStates(mus["f64"], "Jackson")
+    plan = spc.build_transform_plan(TransformConfig(), a, b, 300, "maxdos")
+    spc.getbounds(plan, a, b, rho, "mean")
+    spc.getorderofexpansion(plan)
+    coeffs = spc.normalized_coefficients(plan.coefficients, plan.target, plan.order)
+    out.update({"target": plan.target, "order": plan.order,
+                "maxdim": PolfedDefaults.expectedkrylovdim(300, 32, 1.25)})
+    q = torch.randn((32, op.D), dtype=torch.float64, device="cuda",
+                    generator=torch.Generator(device="cuda").manual_seed(9))
+    y = torch.empty_like(q)
+    iters = -(-out["maxdim"] // 32)
+    for dt in ("f32", "f64"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.filter(coeffs, q, y, dtype=dt)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        steps = plan.order + 1
+        out[f"filter_{dt}"] = {"time_s": t, "us_per_step": t / steps * 1e6,
+                               "algorithmic_GBps": 4 * 32 * op.D * (4 if dt == "f32" else 8)
+                               * steps / t / 1e9,
+                               "projected_solve_filter_s": t * iters}
+        log(f"filter {dt}: {t:.1f} s ({t / steps * 1e6:.1f} us/step)")
+    print(json.dumps(out), flush=True)
+
+
 def copy_calibration():
     """Plain device copy of a known size (256 MiB) for calibrating the
     FETCH_SIZE / WRITE_SIZE byte counters."""
@@ -246,6 +446,13 @@ def main():
                     help="filter arithmetic for --filter-only")
     ap.add_argument("--reorth-only", type=int, default=0, metavar="CALLS",
                     help="profiling driver: time CALLS CGS2 reorthogonalizations")
+    ap.add_argument("--workload", default="xxz-l20",
+                    choices=["xxz-l20", "qsun-batch", "xxz-l22", "j1j2-l24"],
+                    help="xxz-l20: the bench line; qsun-batch: 64 Quantum-Sun instances; "
+                         "xxz-l22: FP32/FP64 precision report; j1j2-l24: KPM and filter at L=24")
+    ap.add_argument("--instances", type=int, default=64, help="qsun-batch: number of instances")
+    ap.add_argument("--plain", action="store_true",
+                    help="qsun-batch: also time the plain FP64 path, one instance at a time")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="profiling driver: one 256 MiB device copy, five times")
     args = ap.parse_args()
@@ -261,6 +468,12 @@ def main():
         return filter_only(args)
     if args.reorth_only:
         return reorth_only(args)
+    if args.workload == "qsun-batch":
+        return qsun_batch(args)
+    if args.workload == "xxz-l22":
+        return xxz_l22(args)
+    if args.workload == "j1j2-l24":
+        return j1j2_l24(args)
     import numpy as np
     import torch
     from polfed_amd.device import Device

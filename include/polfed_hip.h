@@ -150,14 +150,22 @@ int pf_filter(pf_op* op, int32_t dtype, int32_t polytype, const double* coeffs, 
 
 /* ------------------------------------------------------------- tuning */
 
-enum { PF_TUNE_XCD_REMAP = 1, PF_TUNE_VEC = 2, PF_TUNE_RESET = 3, PF_TUNE_NONTEMPORAL = 4 };
+enum {
+  PF_TUNE_XCD_REMAP = 1,
+  PF_TUNE_VEC = 2,
+  PF_TUNE_RESET = 3,
+  PF_TUNE_NONTEMPORAL = 4,
+  PF_TUNE_PBLOCKS = 5
+};
 
 /* Force a launch knob of the row-interleaved step kernels for this operator:
  * PF_TUNE_XCD_REMAP (-1 default, 0 off, 1 on: contiguous row ranges per XCD),
  * PF_TUNE_VEC (0 default, or 1/2/4/8/16/32 elements per lane),
  * PF_TUNE_NONTEMPORAL (-1 default, 0/1: non-temporal hints on the streamed
- * vectors), PF_TUNE_RESET (drop forced and autotuned settings).  No reference counterpart: launch tuning
- * of this library. */
+ * vectors), PF_TUNE_PBLOCKS (-1 default, 0: one block per row group, n: a
+ * persistent grid of n blocks per CU, each over a contiguous range of rows),
+ * PF_TUNE_RESET (drop forced and autotuned settings).  No reference
+ * counterpart: launch tuning of this library. */
 int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value);
 
 /* Autotuner: times `nsteps` fused recurrence steps for each candidate launch

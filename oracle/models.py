@@ -70,6 +70,11 @@ def lup_basis(L: int, Lup: int) -> np.ndarray:
     with popcount Lup (brute-force enumeration)."""
     if not 0 <= Lup <= L:
         raise ValueError(f"Lup must satisfy 0 <= Lup <= L; got Lup={Lup} for L={L}.")
+    if L > 22:
+        # sparse sectors of long chains: the same set from the Lup-subsets of
+        # the sites, sorted ascending (agreement with the enumeration below is
+        # checked for L <= 16 in the oracle tests)
+        return lup_basis_combinations(L, Lup)
     states = np.arange(1 << L, dtype=np.int64)
     pc = np.zeros_like(states)
     x = states.copy()
@@ -133,6 +138,15 @@ def spin_half_parts(L, bonds, couplings, anisotropies, fields, Lup):
         cols.append(col_idx[anti])
         vals.append(np.full(int(anti.sum()), J / 2))
     return basis, diag, rows, cols, vals
+
+
+def lup_basis_combinations(L: int, Lup: int) -> np.ndarray:
+    """Ascending L-bit states with popcount Lup built from the Lup-subsets of
+    the sites (for sectors whose 2^L enumeration is too large)."""
+    import itertools
+    out = np.fromiter((sum(1 << p for p in c) for c in itertools.combinations(range(L), Lup)),
+                      dtype=np.int64)
+    return np.sort(out)
 
 
 def spin_half_hamiltonian(L, bonds, couplings, anisotropies, fields, Lup, use_sparse=True):

@@ -30,6 +30,7 @@ PF_TUNE_XCD_REMAP = 1
 PF_TUNE_VEC = 2
 PF_TUNE_RESET = 3
 PF_TUNE_NONTEMPORAL = 4
+PF_TUNE_PBLOCKS = 5
 
 PF_BATCH_GRAPH = 1
 PF_BATCH_GRAPH_STEPS = 2

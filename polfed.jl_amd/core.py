@@ -289,7 +289,8 @@ def polfed_batch(ops, x0s, howmany: int, target, *, produce_report: bool = False
                  mapping: Optional[MappingConfig] = None,
                  transform: Optional[TransformConfig] = None,
                  fact: Optional[FactorizationConfig] = None, dos: Optional[DoSConfig] = None,
-                 probes=None, device: Optional[Device] = None, autotune: bool = True):
+                 probes=None, device: Optional[Device] = None, autotune: bool = True,
+                 progress=None):
     """``polfed`` on many Quantum-Sun instances at once (disorder realizations,
     possibly of different sizes).  Every instance runs the reference algorithm
     exactly as ``polfed`` does -- its own bounds, DoS, interval, order and
@@ -301,6 +302,8 @@ def polfed_batch(ops, x0s, howmany: int, target, *, produce_report: bool = False
     Returns a list of ``(vals, vecs)`` per instance, or
     ``(list of (vals, vecs, report), BatchReport)`` with ``produce_report``.
     ``probes``: None or a list of per-instance D_i x R probe blocks.
+    ``progress``: optional callable(str) for one line per set-up instance and
+    per ten batched filter applications.
     """
     from .batch import QSunBatch
     from .lanczos import lanczos_coroutine
@@ -316,9 +319,12 @@ def polfed_batch(ops, x0s, howmany: int, target, *, produce_report: bool = False
         raise ValueError("polfed_batch: one initial state per operator")
     dev = device or ops[0].device or default_device()
     t_all = time.perf_counter()
-    preps = [_prepare(op, x0, howmany, target, mapping, transform, fact, dos,
-                      None if probes is None else probes[i], dev)
-             for i, (op, x0) in enumerate(zip(ops, x0s))]
+    preps = []
+    for i, (op, x0) in enumerate(zip(ops, x0s)):
+        preps.append(_prepare(op, x0, howmany, target, mapping, transform, fact, dos,
+                              None if probes is None else probes[i], dev))
+        if progress:
+            progress(f"set-up {i + 1}/{n}: D={op.D} order={preps[-1].plan.order}")
     s = preps[0].s
     if any(p.s != s for p in preps):
         raise ValueError("polfed_batch: every instance must use the same block size")
@@ -356,6 +362,8 @@ def polfed_batch(ops, x0s, howmany: int, target, *, produce_report: bool = False
         events.append((e0, e1))
         calls += 1
         steps += int(max(batch.orders[i] for i in pending)) + 1
+        if progress and calls % 10 == 0:
+            progress(f"batched filter {calls}: {len(pending)} instances running")
         for i in list(pending):
             try:
                 pending[i] = gens[i].send(None)

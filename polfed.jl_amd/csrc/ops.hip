@@ -73,23 +73,41 @@ __device__ __forceinline__ uint32_t group_bcast(uint32_t v) {
   }
 }
 
+// Where a lane finds neighbour ranks: the two rank tables staged in LDS
+// (TBL) or the O(1) update from the binomial table; bond masks in LDS.
+struct RankCtx {
+  const uint32_t* C;   // binomials C(p, t) (!TBL)
+  const uint16_t* TA;  // rank of the low m bits (TBL)
+  const uint32_t* TB;  // segment offset of the high L-m bits (TBL)
+  const uint32_t* M;   // bond masks (1 << lo) | (1 << hi)
+  uint32_t maskA;
+  int m;
+};
+
+// Rank of the neighbour across bond b, or r when its two spins are parallel.
+template <bool TBL>
+__device__ __forceinline__ uint32_t one_rank(const RankCtx& R, int b, uint32_t st, uint32_t r) {
+  const uint32_t mk = R.M[b];
+  if (popc32(st & mk) != 1) return r;
+  if constexpr (TBL) {
+    const uint32_t x = st ^ mk;
+    return (uint32_t)R.TA[x & R.maskA] + R.TB[x >> R.m];
+  } else {
+    return neighbor_rank(st, r, ctz32(mk), 31 - __clz(mk), R.C);
+  }
+}
+
 // Ranks of the neighbours of bonds b0 + j, j < NBAT, as seen by every lane of
 // the row group: with GM in {1,2,4,8} lane g computes the bonds j = g + GM*u
 // and the group shares them; GM = 0 (any other group size) computes all NBAT.
-// A bond whose two spins are parallel yields the row's own rank r.
-template <int GM, int NBAT>
-__device__ __forceinline__ void bond_ranks(const ChainParams& P, int b0, uint32_t st, uint32_t r,
-                                           const uint32_t* C, int g, uint32_t (&out)[NBAT]) {
+template <int GM, int NBAT, bool TBL>
+__device__ __forceinline__ void bond_ranks(const RankCtx& R, int nb, int b0, uint32_t st,
+                                           uint32_t r, int g, uint32_t (&out)[NBAT]) {
   if constexpr (GM == 0) {
 #pragma unroll
     for (int j = 0; j < NBAT; ++j) {
       const int b = b0 + j;
-      uint32_t rn = r;
-      if (b < P.nb) {  // uniform
-        const int lo = P.lo[b], hi = P.hi[b];
-        if (((st >> lo) ^ (st >> hi)) & 1u) rn = neighbor_rank(st, r, lo, hi, C);
-      }
-      out[j] = rn;
+      out[j] = b < nb ? one_rank<TBL>(R, b, st, r) : r;  // b < nb is uniform
     }
   } else {
     constexpr int UB = NBAT / GM;
@@ -97,12 +115,7 @@ __device__ __forceinline__ void bond_ranks(const ChainParams& P, int b0, uint32_
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
       const int b = b0 + g + GM * u;
-      uint32_t rn = r;
-      if (b < P.nb) {
-        const int lo = P.lo[b], hi = P.hi[b];  // per-lane bond: scalar table lookup
-        if (((st >> lo) ^ (st >> hi)) & 1u) rn = neighbor_rank(st, r, lo, hi, C);
-      }
-      mine[u] = rn;
+      mine[u] = b < nb ? one_rank<TBL>(R, b, st, r) : r;
     }
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
@@ -127,23 +140,43 @@ __device__ __forceinline__ void bond_ranks(const ChainParams& P, int b0, uint32_
 // out = cx*X + ch*((H - shift)*scale) b2 + cb*b3 on rows of the Lup sector.
 // The G = s/V lanes of a row split the neighbour-rank work (GM = G) and each
 // gathers its V columns of every flippable neighbour; NBAT neighbours are
-// loaded before any is accumulated.
-template <typename T, int V, int GM>
+// loaded before any is accumulated.  Block p (XCD-grouped) owns the
+// contiguous row groups [p*n/nblk, (p+1)*n/nblk): with a persistent grid the
+// rank tables are staged in LDS once per block, not once per row group.
+template <typename T, int V, int GM, bool TBL>
 __global__ __launch_bounds__(256) void chain_step_kernel(
     const ChainParams P, const uint32_t* __restrict__ states, const uint32_t* __restrict__ binom,
-    const double* __restrict__ diag, const Affine af, const int flags, const T cx, const T ch,
-    const T cb, const T* __restrict__ X, const T* __restrict__ B2, const T* __restrict__ B3,
+    const uint4* __restrict__ rtab, const int rtab_vec, const double* __restrict__ diag,
+    const Affine af, const int flags, const T cx, const T ch, const T cb,
+    const T* __restrict__ X, const T* __restrict__ B2, const T* __restrict__ B3,
     void* __restrict__ OUT, const Layout lay, const int64_t out_ld, double* __restrict__ dot_part,
     const T* __restrict__ Pr, const int64_t ngroups) {
   typedef typename VT<T, V>::t vt;
   constexpr int GE = GM < 1 ? 1 : GM;
   constexpr int NB0 = (V * (int)sizeof(T) <= 32) ? 8 : 4;
   constexpr int NBAT = NB0 < GE ? GE : NB0;
-  __shared__ uint32_t sC[32 * kRankT];
+  extern __shared__ uint4 tab_lds[];
+  __shared__ uint32_t sM[kMaxBonds];
   __shared__ T sW[kMaxBonds];
   __shared__ double red[4];
-  for (int i = threadIdx.x; i < 32 * kRankT; i += blockDim.x) sC[i] = binom[i];
-  for (int i = threadIdx.x; i < P.nb; i += blockDim.x) sW[i] = (T)(P.bval[i] * af.scale);
+  RankCtx R{};
+  R.M = sM;
+  if constexpr (TBL) {
+    for (int i = threadIdx.x; i < rtab_vec; i += blockDim.x) tab_lds[i] = rtab[i];
+    R.TA = reinterpret_cast<const uint16_t*>(tab_lds);
+    R.TB = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(tab_lds) +
+                                             (2u << P.tsplit));
+    R.maskA = (1u << P.tsplit) - 1u;
+    R.m = P.tsplit;
+  } else {
+    __shared__ uint32_t sC[32 * kRankT];
+    for (int i = threadIdx.x; i < 32 * kRankT; i += blockDim.x) sC[i] = binom[i];
+    R.C = sC;
+  }
+  for (int i = threadIdx.x; i < P.nb; i += blockDim.x) {
+    sW[i] = (T)(P.bval[i] * af.scale);
+    sM[i] = (1u << P.lo[i]) | (1u << P.hi[i]);
+  }
   __syncthreads();
 
   const int rloc = threadIdx.x / lay.G;
@@ -153,9 +186,10 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
   const int out_col0 = (int)blockIdx.y + g * V;
   double dacc = 0.0;
 
-  const bool remap = lay.xcd && (int64_t)gridDim.x == ngroups;
-  for (int64_t it = blockIdx.x; it < ngroups; it += gridDim.x) {
-    const int64_t grp = remap ? xcd_block(it, ngroups) : it;
+  const int64_t nblk = gridDim.x;
+  const int64_t p = lay.xcd ? xcd_block(blockIdx.x, nblk) : (int64_t)blockIdx.x;
+  const int64_t g_end = (p + 1) * ngroups / nblk;  // balanced contiguous ranges
+  for (int64_t grp = p * ngroups / nblk; grp < g_end; ++grp) {
     const int64_t row = grp * lay.rows_per_block + rloc;
     if (!lane_ok || row >= P.D) continue;  // whole row groups drop out together
     vt h = (vt)(T)0;
@@ -169,7 +203,7 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
       vt tot = (vt)(T)0;
       for (int b0 = 0; b0 < P.nb; b0 += NBAT) {
         uint32_t rk[NBAT];
-        bond_ranks<GM, NBAT>(P, b0, st, r32, sC, g, rk);
+        bond_ranks<GM, NBAT, TBL>(R, P.nb, b0, st, r32, g, rk);
         vt x[NBAT];
 #pragma unroll
         for (int j = 0; j < NBAT; ++j) {
@@ -272,7 +306,7 @@ static bool valid_vec(int dtype, int s, int V) {
   return s / V <= 256;
 }
 
-Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt) {
+Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt, int pb) {
   Layout l{};
   if (V == 0 || !valid_vec(dtype, s, V)) {
     if (dtype == PF_F64)
@@ -287,7 +321,14 @@ Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt) {
   l.rows_per_block = 256 / l.G;
   l.xcd = xcd;
   l.nt = nt;
+  l.pblocks = pb;
   return l;
+}
+
+// Default persistent grid: spin chains with LDS rank tables stage them once
+// per block, so they run 8 blocks per CU over all row groups.
+static int default_pblocks(const pf_op* op) {
+  return (op->kind == kChain && op->chain.tsplit > 0) ? 8 : 0;
 }
 
 // Configuration for (dtype, s): forced knobs, else the autotuned entry, else defaults.
@@ -298,7 +339,9 @@ Layout tuned_layout(pf_op* op, int dtype, int s) {
   if (op->force.V) c.V = op->force.V;
   if (op->force.xcd >= 0) c.xcd = op->force.xcd;
   if (op->force.nt >= 0) c.nt = op->force.nt;
-  return interleaved_layout(dtype, s, c.V, c.xcd < 0 ? 1 : c.xcd, c.nt < 0 ? 1 : c.nt);
+  if (op->force.pb >= 0) c.pb = op->force.pb;
+  return interleaved_layout(dtype, s, c.V, c.xcd < 0 ? 1 : c.xcd, c.nt < 0 ? 1 : c.nt,
+                            c.pb < 0 ? default_pblocks(op) : c.pb);
 }
 
 Layout colmajor_layout(int64_t ld) {
@@ -313,6 +356,17 @@ Layout colmajor_layout(int64_t ld) {
   return l;
 }
 
+// Blocks of one step launch: one per row group, capped by the persistent grid
+// (pblocks per CU) and, for KPM dot launches, by the number of partial slots.
+static int64_t step_grid_x(const pf_op* op, const Layout& lay, int flags, int dot_blocks) {
+  const int64_t ngroups = (op->D + lay.rows_per_block - 1) / lay.rows_per_block;
+  int64_t gx = ngroups;
+  if (lay.pblocks > 0)
+    gx = std::min<int64_t>(gx, (int64_t)lay.pblocks * std::max(1, op->h->prop.multiProcessorCount));
+  if (flags & kDot) gx = std::min<int64_t>(gx, dot_blocks);
+  return gx;
+}
+
 template <typename T, int V>
 static void launch_typed(pf_op* op, int flags, const Layout& lay, int ncols_grid, Affine af,
                          double cx, double ch, double cb, const void* X, const void* B2,
@@ -321,8 +375,7 @@ static void launch_typed(pf_op* op, int flags, const Layout& lay, int ncols_grid
   PF_REQUIRE((uint64_t)op->D * (uint64_t)lay.rs < ((uint64_t)1 << 32),
              "block too large for 32-bit row offsets");
   const int64_t ngroups = (op->D + lay.rows_per_block - 1) / lay.rows_per_block;
-  int64_t gx = ngroups;
-  if (flags & kDot) gx = std::min<int64_t>(ngroups, dot_blocks);
+  const int64_t gx = step_grid_x(op, lay, flags, dot_blocks);
   PF_REQUIRE(gx < (int64_t)1 << 31, "grid too large");
   dim3 grid((unsigned)gx, (unsigned)ncols_grid), block(256);
   if (lay.nt) flags |= kStreamNT;
@@ -331,17 +384,28 @@ static void launch_typed(pf_op* op, int flags, const Layout& lay, int ncols_grid
   const T* B3t = reinterpret_cast<const T*>(B3);
   const T* Pt = reinterpret_cast<const T*>(P);
   if (op->kind == kChain) {
-#define PF_CHAIN_LAUNCH(GMV)                                                                   \
-  hipLaunchKernelGGL((chain_step_kernel<T, V, GMV>), grid, block, 0, st, op->chain, op->states, \
-                     op->binom, op->diag, af, flags, (T)cx, (T)ch, (T)cb, Xt, B2t, B3t, out,   \
-                     lay, out_ld, dot_part, Pt, ngroups)
-    switch (lay.G) {
-      case 1: PF_CHAIN_LAUNCH(1); break;
-      case 2: PF_CHAIN_LAUNCH(2); break;
-      case 4: PF_CHAIN_LAUNCH(4); break;
-      case 8: PF_CHAIN_LAUNCH(8); break;
-      default: PF_CHAIN_LAUNCH(0); break;
+    const bool tbl = op->chain.tsplit > 0 && op->rtab;
+    const uint4* rt = reinterpret_cast<const uint4*>(op->rtab);
+    const int rvec = (int)(op->rtab_bytes / 16);
+    const size_t lds = tbl ? op->rtab_bytes : 0;
+#define PF_CHAIN_LAUNCH(GMV, TB)                                                               \
+  hipLaunchKernelGGL((chain_step_kernel<T, V, GMV, TB>), grid, block, lds, st, op->chain,      \
+                     op->states, op->binom, rt, rvec, op->diag, af, flags, (T)cx, (T)ch, (T)cb, \
+                     Xt, B2t, B3t, out, lay, out_ld, dot_part, Pt, ngroups)
+#define PF_CHAIN_G(TB)                      \
+  switch (lay.G) {                          \
+    case 1: PF_CHAIN_LAUNCH(1, TB); break;  \
+    case 2: PF_CHAIN_LAUNCH(2, TB); break;  \
+    case 4: PF_CHAIN_LAUNCH(4, TB); break;  \
+    case 8: PF_CHAIN_LAUNCH(8, TB); break;  \
+    default: PF_CHAIN_LAUNCH(0, TB); break; \
+  }
+    if (tbl) {
+      PF_CHAIN_G(true);
+    } else {
+      PF_CHAIN_G(false);
     }
+#undef PF_CHAIN_G
 #undef PF_CHAIN_LAUNCH
   } else {
     hipLaunchKernelGGL((qsun_step_kernel<T, V>), grid, block, 0, st, op->qsun, af, flags, (T)cx,
@@ -649,7 +713,47 @@ extern "C" int pf_op_create_spin_chain(pf_handle* h, int32_t L, int32_t Lup, int
     }
     std::vector<uint32_t> C(32 * kRankT);
     fill_binomials(C.data());
+    // Two-table rank: with a = low m bits, b = high L-m bits of a state,
+    // rank = TA[a] + TB[b], TA[a] = sum_t C(p_t, t+1) over the set bits of a
+    // and TB[b] = sum_t' C(m + q_t', nA + t' + 1) over the set bits q of b,
+    // nA = Lup - |b| (the particles of a come first in the combinadic sum).
+    std::vector<unsigned char> rtab;
+    if (L <= kMaxTableL) {
+      const int m = (L + 1) / 2;
+      const uint32_t nA = 1u << m, nB = 1u << (L - m);
+      const size_t bytes = ((size_t)nA * 2 + (size_t)nB * 4 + 15) & ~(size_t)15;
+      rtab.assign(bytes, 0);
+      uint16_t* TA = reinterpret_cast<uint16_t*>(rtab.data());
+      uint32_t* TB = reinterpret_cast<uint32_t*>(rtab.data() + (size_t)nA * 2);
+      for (uint32_t a = 0; a < nA; ++a) TA[a] = (uint16_t)full_rank(a, C.data());
+      for (uint32_t b = 0; b < nB; ++b) {
+        const int na = Lup - popc32(b);
+        if (na < 0 || na > m) continue;
+        uint64_t off = 0;
+        int t = 0;
+        for (uint32_t x = b; x; x &= x - 1u) {
+          off += C[(m + ctz32(x)) * kRankT + na + t + 1];
+          ++t;
+        }
+        TB[b] = (uint32_t)off;
+      }
+      for (uint64_t r = 0; r < D; ++r) {
+        const uint32_t st = states[r];
+        if ((uint64_t)TA[st & (nA - 1)] + TB[st >> m] != r) {
+          delete op;
+          throw Error{PF_ERR_INVALID, "spin chain: rank table check failed"};
+        }
+      }
+      P.tsplit = m;
+    } else {
+      P.tsplit = 0;
+    }
     try {
+      if (!rtab.empty()) {
+        PF_HIP(hipMalloc(&op->rtab, rtab.size()));
+        PF_HIP(hipMemcpy(op->rtab, rtab.data(), rtab.size(), hipMemcpyHostToDevice));
+        op->rtab_bytes = rtab.size();
+      }
       PF_HIP(hipMalloc(&op->states, D * sizeof(uint32_t)));
       PF_HIP(hipMalloc(&op->binom, C.size() * sizeof(uint32_t)));
       PF_HIP(hipMalloc(&op->diag, D * sizeof(double)));
@@ -726,6 +830,7 @@ extern "C" int pf_op_destroy(pf_op* op) {
   if (op->states) (void)hipFree(op->states);
   if (op->binom) (void)hipFree(op->binom);
   if (op->diag) (void)hipFree(op->diag);
+  if (op->rtab) (void)hipFree(op->rtab);
   op->fbuf.release();
   op->kpart.release();
   delete op;
@@ -801,7 +906,8 @@ extern "C" int pf_apply(pf_op* op, int32_t dtype, int32_t rescaled, const void* 
     PF_REQUIRE(ncols >= 1 && ncols <= 65535, "apply: 1 <= ncols <= 65535");
     PF_REQUIRE(ldx == ldy && ldx >= op->D, "apply: X and Y need the same leading dimension >= D");
     PF_REQUIRE(X != Y, "apply: X and Y must not alias");
-    const Layout lay = colmajor_layout(ldx);
+    Layout lay = colmajor_layout(ldx);
+    lay.pblocks = default_pblocks(op);
     launch_step(op, dtype, kHasH, lay, ncols, op->affine(rescaled != 0), 0.0, 1.0, 0.0, nullptr, X,
                 nullptr, Y, 0, S(stream));
   });
@@ -819,7 +925,9 @@ extern "C" int pf_clenshaw_recurrence(pf_op* op, int32_t dtype, void* b1, const 
       PF_REQUIRE(b3 != nullptr, "recurrence: b3 is null but beta != 0");
       flags |= kHasB3;
     }
-    launch_step(op, dtype, flags, colmajor_layout(ld), ncols, op->affine(true), c, alpha, beta, X,
+    Layout lay = colmajor_layout(ld);
+    lay.pblocks = default_pblocks(op);
+    launch_step(op, dtype, flags, lay, ncols, op->affine(true), c, alpha, beta, X,
                 b2, b3, b1, 0, S(stream));
   });
 }
@@ -836,7 +944,9 @@ extern "C" int pf_clenshaw_finalsum(pf_op* op, int32_t dtype, const void* b2, co
       PF_REQUIRE(b3 != nullptr, "finalsum: b3 is null");
       flags |= kHasB3;
     }
-    launch_step(op, dtype, flags, colmajor_layout(ld), ncols, op->affine(true), c0 * phi0, phi1,
+    Layout lay = colmajor_layout(ld);
+    lay.pblocks = default_pblocks(op);
+    launch_step(op, dtype, flags, lay, ncols, op->affine(true), c0 * phi0, phi1,
                 beta1 * phi0, X, b2, b3, Y, 0, S(stream));
   });
 }
@@ -975,8 +1085,7 @@ extern "C" int pf_kpm_moments(pf_op* op, int32_t dtype, int32_t N, int32_t R, co
         tn = tmp;
       }
       // every step wrote nblk block partials at stride dot_blocks
-      const int64_t ngroups = (op->D + lay.rows_per_block - 1) / lay.rows_per_block;
-      const int nblk = (int)std::min<int64_t>(ngroups, dot_blocks);
+      const int nblk = (int)step_grid_x(op, lay, kDot, dot_blocks);
       hipLaunchKernelGGL(kpm_reduce_kernel, dim3((N + 255) / 256), dim3(256), 0, st, part,
                          dot_blocks, nblk, N, mu_dev);
       PF_HIP(hipGetLastError());
@@ -1003,6 +1112,9 @@ extern "C" int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value) {
     } else if (key == PF_TUNE_NONTEMPORAL) {
       PF_REQUIRE(value >= -1 && value <= 1, "tuning: non-temporal is -1, 0 or 1");
       op->force.nt = value;
+    } else if (key == PF_TUNE_PBLOCKS) {
+      PF_REQUIRE(value >= -1 && value <= 64, "tuning: persistent blocks per CU is -1..64");
+      op->force.pb = value;
     } else if (key == PF_TUNE_RESET) {
       op->force = TuneCfg{};
       op->tuned.clear();
@@ -1041,19 +1153,24 @@ extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nste
     };
     std::vector<Cand> cands;
     const int vs[6] = {1, 2, 4, 8, 16, 32};
+    const int pbs[3] = {default_pblocks(op), 4, 0};
     for (int vi = 0; vi < 6; ++vi) {
       if (!valid_vec(dtype, ncols, vs[vi])) continue;
-      for (int cfg = 0; cfg < 4; ++cfg) {
-        TuneCfg c;
-        c.V = vs[vi];
-        c.xcd = cfg & 1;
-        c.nt = cfg >> 1;
-        cands.push_back({c, 1e300});
+      for (int pi = 0; pi < 3; ++pi) {
+        if (pi > 0 && pbs[pi] == pbs[0]) continue;
+        for (int x = 1; x >= 0; --x) {
+          TuneCfg c;
+          c.V = vs[vi];
+          c.xcd = x;
+          c.nt = 1;
+          c.pb = pbs[pi];
+          cands.push_back({c, 1e300});
+        }
       }
     }
     for (int round = 0; round < 3; ++round) {
       for (auto& cd : cands) {
-        const Layout lay = interleaved_layout(dtype, ncols, cd.c.V, cd.c.xcd, cd.c.nt);
+        const Layout lay = interleaved_layout(dtype, ncols, cd.c.V, cd.c.xcd, cd.c.nt, cd.c.pb);
         for (int w = 0; w < 2; ++w)
           launch_step(op, dtype, kHasX | kHasH | kHasB3, lay, ncols, af, 0.5, 2.0, -1.0, b[0],
                       b[1], b[2], b[3], 0, st);

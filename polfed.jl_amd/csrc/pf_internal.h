@@ -94,7 +94,13 @@ struct ChainParams {
   int8_t lo[kMaxBonds], hi[kMaxBonds];  // 0-based bit positions, lo < hi
   double cval[kMaxClasses];             // unscaled off-diagonal value J/2 of each class
   double bval[kMaxBonds];               // cval of each bond's class
+  // Two-table rank: rank(s) = TA[s & (2^m - 1)] + TB[s >> m] (m = tsplit,
+  // 0 = tables not built, the kernels then use the O(1) neighbour update).
+  int32_t tsplit;
 };
+
+// Largest L whose rank tables are staged in LDS (2^m u16 + 2^(L-m) u32).
+constexpr int kMaxTableL = 26;
 
 // Quantum-Sun operator parameters (spin 1/2, full space).
 struct QsunParams {
@@ -132,6 +138,7 @@ struct Layout {
   int V;           // elements per lane
   int xcd;         // 1: map consecutive row groups to the same XCD (L2 locality)
   int nt;          // 1: non-temporal hints on streamed loads/stores
+  int pblocks;     // persistent grid: blocks per CU (0: one block per row group)
 };
 
 // Launch configuration of the row-interleaved step kernel for one (dtype, s).
@@ -139,6 +146,7 @@ struct TuneCfg {
   int V = 0;    // elements per lane (0 = default)
   int xcd = -1; // XCD-contiguous row mapping (-1 = default)
   int nt = -1;  // non-temporal streamed accesses (-1 = default)
+  int pb = -1;  // persistent blocks per CU (-1 = default)
 };
 
 }  // namespace pf
@@ -161,6 +169,8 @@ struct pf_op {
   uint32_t* states = nullptr;  // spin chain: ascending basis states
   uint32_t* binom = nullptr;   // spin chain: C(p, t), p < 32, t < kRankT
   double* diag = nullptr;      // spin chain: unscaled diagonal, reference summation order
+  void* rtab = nullptr;        // spin chain: u16 TA[2^m] then u32 TB[2^(L-m)] (rank tables)
+  size_t rtab_bytes = 0;       // bytes of rtab (dynamic LDS of the step kernel)
   pf::DevBuf fbuf;             // filter / KPM row-interleaved work blocks
   pf::TuneCfg force;           // pf_op_set_tuning overrides
   std::map<std::pair<int, int>, pf::TuneCfg> tuned;  // autotuned (dtype, s) configs
@@ -184,7 +194,7 @@ void launch_step(pf_op* op, int dtype, int flags, const Layout& lay, int ncols, 
                  void* out, int64_t out_ld, hipStream_t st, double* dot_part = nullptr,
                  const void* P = nullptr, int dot_blocks = 0);
 
-Layout interleaved_layout(int dtype, int s, int V = 0, int xcd = 1, int nt = 1);
+Layout interleaved_layout(int dtype, int s, int V = 0, int xcd = 1, int nt = 1, int pb = 0);
 Layout tuned_layout(pf_op* op, int dtype, int s);
 Layout colmajor_layout(int64_t ld);
 

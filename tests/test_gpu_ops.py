@@ -83,6 +83,46 @@ def _rel(a, b):
     return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
 
 
+def test_long_chain_without_rank_tables(dev):
+    """L = 28 > 26: no LDS rank tables, the step kernel uses the O(1)
+    neighbour-rank update (with the O(Lup) periodic bond); apply and a filter
+    against the oracle's sparse matrix, connectivity bit-exact."""
+    from polfed_amd.operators import SpinChainOperator
+    L, Lup = 28, 3
+    bonds, Js, Ds = om.j1j2_bonds(L, 1.0, 0.5, 1.0, 0.7, "periodic")
+    h = np.random.default_rng(12).uniform(-1, 1, L)
+    op = SpinChainOperator(L, Lup, bonds, Js, Ds, h, device=dev)
+    H = om.spin_half_hamiltonian(L, bonds, Js, Ds, h, Lup)
+    assert op.D == H.shape[0] == 3276
+    cols, vals, counts, diag = op.connectivity()
+    assert np.array_equal(diag, H.diagonal())
+    Hd = _dense_rows(cols, vals, counts, op.D) + sp.diags(diag)
+    assert abs(Hd - H).max() == 0.0
+    X = np.random.default_rng(2).standard_normal((op.D, 4))
+    assert _rel(dev.to_host(op.apply(dev.to_device(X))), H @ X) < 1e-13
+    w = np.linalg.eigvalsh(H.toarray())
+    op.set_rescale(w[0], w[-1])
+    a, b = (w[-1] - w[0]) / 2, (w[-1] + w[0]) / 2
+    Ht = (H - b * sp.identity(op.D)) / a
+    c = oc.filter_coefficients(0.3, 80)
+    assert _rel(dev.to_host(op.filter(c, dev.to_device(X))), oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)) < 1e-12
+
+
+@pytest.mark.parametrize("pblocks", [0, 1, 8])
+def test_filter_persistent_grid_settings_agree(dev, pblocks):
+    """The persistent-grid knob changes only the block -> row mapping."""
+    op, H = _chain(dev, *CHAINS[2][1:])
+    w = np.linalg.eigvalsh(H.toarray())
+    op.set_rescale(w[0], w[-1])
+    a, b = (w[-1] - w[0]) / 2, (w[-1] + w[0]) / 2
+    Ht = (H - b * sp.identity(H.shape[0])) / a
+    c = oc.filter_coefficients(-0.2, 150)
+    X = np.random.default_rng(7).standard_normal((H.shape[0], 8))
+    op.set_tuning(pblocks=pblocks)
+    Y = dev.to_host(op.filter(c, dev.to_device(X)))
+    assert _rel(Y, oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)) < 1e-12
+
+
 @pytest.mark.parametrize("s", [1, 3, 8])
 @pytest.mark.parametrize("kind", ["chain", "qsun"])
 def test_apply_matches_sparse(dev, kind, s):

@@ -90,6 +90,11 @@ def test_model_argument_validation():
         om.j1j2_hamiltonian(4, 2, 1.0, 0.5, 1.0, 0.8, 0.0, boundary="twisted")
 
 
+@pytest.mark.parametrize("L,Lup", [(4, 2), (9, 0), (12, 5), (16, 8), (16, 3)])
+def test_basis_from_combinations_matches_enumeration(L, Lup):
+    assert np.array_equal(om.lup_basis_combinations(L, Lup), om.lup_basis(L, Lup))
+
+
 @pytest.mark.parametrize("L", [4, 9, 14, 16])
 def test_combinadic_rank_is_ascending_index(L):
     """closed-form rank == position in the brute-force ascending basis."""

@@ -155,7 +155,8 @@ enum {
   PF_TUNE_VEC = 2,
   PF_TUNE_RESET = 3,
   PF_TUNE_NONTEMPORAL = 4,
-  PF_TUNE_PBLOCKS = 5
+  PF_TUNE_PBLOCKS = 5,
+  PF_TUNE_WIDE = 6
 };
 
 /* Force a launch knob of the row-interleaved step kernels for this operator:
@@ -164,6 +165,8 @@ enum {
  * PF_TUNE_NONTEMPORAL (-1 default, 0/1: non-temporal hints on the streamed
  * vectors), PF_TUNE_PBLOCKS (-1 default, 0: one block per row group, n: a
  * persistent grid of n blocks per CU, each over a contiguous range of rows),
+ * PF_TUNE_WIDE (-1 default, 0/1: twice the neighbour gathers in flight per
+ * batch in the spin-chain kernel with rank tables),
  * PF_TUNE_RESET (drop forced and autotuned settings).  No reference
  * counterpart: launch tuning of this library. */
 int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value);

@@ -31,6 +31,7 @@ PF_TUNE_VEC = 2
 PF_TUNE_RESET = 3
 PF_TUNE_NONTEMPORAL = 4
 PF_TUNE_PBLOCKS = 5
+PF_TUNE_WIDE = 6
 
 PF_BATCH_GRAPH = 1
 PF_BATCH_GRAPH_STEPS = 2

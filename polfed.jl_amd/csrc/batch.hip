@@ -27,6 +27,7 @@
 
 #include "pf_device.h"
 #include "pf_internal.h"
+#include "pf_qsun.h"
 
 namespace pf {
 
@@ -37,9 +38,10 @@ struct BatchInst {
   int64_t row0;      // first row of this instance in the batch row space
   int64_t coef_off;  // first filter coefficient of this instance
   double scale, shift;
+  const double* qdiag;  // the operator's diagonal tables (Dlo then Dhi)
+  int32_t lb;           // low bits of the diagonal split
   uint32_t flip[kMaxLoc];
   double coup[kMaxLoc];
-  double h[kMaxLoc];
   double G[(1 << kMaxGrain) * (1 << kMaxGrain)];
 };
 
@@ -85,16 +87,19 @@ template <typename T, int V, int RPT>
 __device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& I,
                                            const BatchDyn& dy, int k, int64_t lb) {
   typedef typename VT<T, V>::t vt;
-  __shared__ T sG[(1 << kMaxGrain) * (1 << kMaxGrain)];
+  __shared__ T sG[kMaxDg * kMaxDg];
+  __shared__ T sC[kMaxLoc];
+  __shared__ uint32_t sF[kMaxLoc];
   const int dg = 1 << I.Lg;
-  for (int t = threadIdx.x; t < dg * dg; t += blockDim.x) sG[t] = (T)(I.G[t] * I.scale);
+  const uint32_t rsb = (uint32_t)(A.s * sizeof(T));
+  const int rsh = (rsb & (rsb - 1u)) == 0 ? __ffs(rsb) - 1 : -1;
+  qsun_stage<T>(I.Lloc, dg, I.flip, I.coup, I.G, I.scale, rsh, sF, sC, sG);
   __syncthreads();
 
   const int g = dy.K - k;  // local step, >= 0
   const bool hasH = g >= 1;
   double cxd, chd, cbd;
   const double ck = A.coefs ? A.coefs[I.coef_off + k] : 0.5;
-  bool hasB3;
   if (k >= 1) {
     double alpha, beta1, dummy;
     poly_ab_dev(A.polytype, k, alpha, dummy);
@@ -105,61 +110,47 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& 
   } else {
     double dummy, beta1;
     poly_ab_dev(A.polytype, 1, dummy, beta1);
-    const double phi1 = A.polytype == PF_HERMITE ? 2.0 : 1.0;
     cxd = ck;  // phi0 = 1 for every family
-    chd = phi1;
+    chd = A.polytype == PF_HERMITE ? 2.0 : 1.0;
     cbd = beta1;
   }
-  hasB3 = g >= 2 && cbd != 0.0;
+  const bool hasB3 = g >= 2 && cbd != 0.0;
   const T cx = (T)cxd, ch = (T)chd, cb = (T)cbd;
 
   const int64_t off = I.row0 * (int64_t)A.s * 8;  // byte offset of this instance's region
   char* bufs[3] = {A.B0, A.B1, A.B2};
   const T* __restrict__ X = reinterpret_cast<const T*>(A.Xb + off);
   T* __restrict__ OUT = reinterpret_cast<T*>(bufs[rot_slot(g)] + off);
-  const T* __restrict__ B2 = reinterpret_cast<const T*>(bufs[rot_slot(g + 2)] + off);  // g-1
+  const char* B2 = bufs[rot_slot(g + 2)] + off;                                       // g-1
   const T* __restrict__ B3 = reinterpret_cast<const T*>(bufs[rot_slot(g + 1)] + off);  // g-2
 
   const int rloc = threadIdx.x / A.G;
   const int gl = threadIdx.x - rloc * A.G;
   if (rloc >= A.rpb) return;
   const int64_t cofs = (int64_t)gl * V;
-  const uint32_t rs32 = (uint32_t)A.s;
-  const T* __restrict__ B2c = B2 + cofs;
-  const double shift = I.shift, scale = I.scale;
+  QsunRow q;
+  q.b2 = B2;
+  q.cofs_b = (uint32_t)(cofs * sizeof(T));
+  q.rsb = rsb;
+  q.rsh = rsh;
+  q.dg = dg;
+  q.nloc = I.Lloc;
+  q.lb = I.lb;
+  q.dlo = I.qdiag;
+  q.dhi = I.qdiag + ((int64_t)1 << I.lb);
+  q.scale = I.scale;
+  q.shift = I.shift;
+  q.sF = sF;
+  q.sC = sC;
+  const bool fixed_a = A.rpb % dg == 0;
+  T wg[kMaxDg];
+  qsun_lane_weights<T>(sG, dg, rloc & (dg - 1), wg);
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
     const int64_t row = (lb * RPT + u) * A.rpb + rloc;
     if (row >= I.D) break;
     vt hv = (vt)(T)0;
-    if (hasH) {
-      const uint32_t r = (uint32_t)row;
-      const int a = (int)(r & (uint32_t)(dg - 1));
-      const uint32_t base = r & ~(uint32_t)(dg - 1);
-      double dd = 0.0;
-      for (int j = 0; j < I.Lloc; ++j) dd += I.h[j] * (((r >> (I.Lg + j)) & 1u) ? 0.5 : -0.5);
-      dd += I.G[a * dg + a];
-      const T d = (T)((dd - shift) * scale);
-      vt acc = (vt)(T)0;
-      for (int b = 0; b < dg; ++b) {
-        const T wb = (b == a) ? d : sG[a * dg + b];
-        acc += wb * vload<T, V>(B2c + (base | (uint32_t)b) * rs32);
-      }
-      for (int j0 = 0; j0 < I.Lloc; j0 += kGatherBatch) {
-        vt x[kGatherBatch];
-        T w[kGatherBatch];
-#pragma unroll
-        for (int q = 0; q < kGatherBatch; ++q) {
-          const int j = j0 + q;
-          const bool ok = j < I.Lloc;
-          w[q] = ok ? (T)(I.coup[j] * scale) : (T)0;
-          x[q] = vload<T, V>(B2c + (ok ? (r ^ I.flip[j]) : r) * rs32);
-        }
-#pragma unroll
-        for (int q = 0; q < kGatherBatch; ++q) acc += w[q] * x[q];
-      }
-      hv = acc;
-    }
+    if (hasH) hv = qsun_hrow<T, V>(q, (uint32_t)row, fixed_a, wg, sG);
     const int64_t o = row * A.s + cofs;
     vt out = cx * vload<T, V>(X + o);
     if (hasH) out += ch * hv;
@@ -294,10 +285,11 @@ struct pf_batch {
       const Affine af = op->affine(true);
       I.scale = af.scale;
       I.shift = af.shift;
+      I.qdiag = op->qdiag;
+      I.lb = op->qlb;
       for (int j = 0; j < kMaxLoc; ++j) {
         I.flip[j] = op->qsun.flip[j];
         I.coup[j] = op->qsun.coup[j];
-        I.h[j] = op->qsun.h[j];
       }
       std::memcpy(I.G, op->qsun.G, sizeof(I.G));
       row0[p] = r0;
@@ -434,6 +426,12 @@ extern "C" int pf_batch_create(pf_handle* h, int32_t n, pf_op* const* ops, int32
       b->layout_blocks();
       b->kstate.get(256);
       b->dyn.get(n * sizeof(BatchDyn));
+      PF_REQUIRE((uint64_t)b->ops[0]->D * ncols * 8 < ((uint64_t)1 << 32) &&
+                     std::all_of(b->ops.begin(), b->ops.end(),
+                                 [&](const pf_op* o) {
+                                   return (uint64_t)o->D * ncols * 8 < ((uint64_t)1 << 32);
+                                 }),
+                 "batch: an instance block exceeds 32-bit byte offsets");
       b->bufs.get(4 * (size_t)b->rows_total * ncols * 8);
     } catch (...) {
       delete b;

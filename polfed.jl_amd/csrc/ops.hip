@@ -20,6 +20,7 @@
 
 #include "pf_device.h"
 #include "pf_internal.h"
+#include "pf_qsun.h"
 
 namespace pf {
 
@@ -143,7 +144,7 @@ __device__ __forceinline__ void bond_ranks(const RankCtx& R, int nb, int b0, uin
 // loaded before any is accumulated.  Block p (XCD-grouped) owns the
 // contiguous row groups [p*n/nblk, (p+1)*n/nblk): with a persistent grid the
 // rank tables are staged in LDS once per block, not once per row group.
-template <typename T, int V, int GM, bool TBL>
+template <typename T, int V, int GM, bool TBL, int NBW>
 __global__ __launch_bounds__(256) void chain_step_kernel(
     const ChainParams P, const uint32_t* __restrict__ states, const uint32_t* __restrict__ binom,
     const uint4* __restrict__ rtab, const int rtab_vec, const double* __restrict__ diag,
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
     const T* __restrict__ Pr, const int64_t ngroups) {
   typedef typename VT<T, V>::t vt;
   constexpr int GE = GM < 1 ? 1 : GM;
-  constexpr int NB0 = (V * (int)sizeof(T) <= 32) ? 8 : 4;
+  constexpr int NB0 = ((V * (int)sizeof(T) <= 32) ? 8 : 4) * NBW;  // gathers in flight
   constexpr int NBAT = NB0 < GE ? GE : NB0;
   extern __shared__ uint4 tab_lds[];
   __shared__ uint32_t sM[kMaxBonds];
@@ -184,22 +185,43 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
   const bool lane_ok = rloc < lay.rows_per_block;
   const int64_t cofs = (int64_t)blockIdx.y * lay.cstride + (int64_t)g * V;
   const int out_col0 = (int)blockIdx.y + g * V;
+  // block-uniform column offset in the 64-bit base, the lane's own in 32 bits
+  const char* b2b = reinterpret_cast<const char*>(B2 + (int64_t)blockIdx.y * lay.cstride);
+  const uint32_t cofs_b = (uint32_t)((int64_t)g * V * sizeof(T));
+  const uint32_t rsb = (uint32_t)(lay.rs * sizeof(T));
+  const int rsh = (rsb & (rsb - 1u)) == 0 ? __ffs(rsb) - 1 : -1;
   double dacc = 0.0;
 
   const int64_t nblk = gridDim.x;
   const int64_t p = lay.xcd ? xcd_block(blockIdx.x, nblk) : (int64_t)blockIdx.x;
   const int64_t g_end = (p + 1) * ngroups / nblk;  // balanced contiguous ranges
-  for (int64_t grp = p * ngroups / nblk; grp < g_end; ++grp) {
-    const int64_t row = grp * lay.rows_per_block + rloc;
+  const bool nt = flags & kStreamNT;
+  const bool use_h = flags & kHasH;
+  // the state and diagonal of the next row group are loaded one iteration
+  // ahead, so their latency overlaps this group's gathers
+  auto row_of = [&](int64_t grp) { return grp * lay.rows_per_block + rloc; };
+  auto row_valid = [&](int64_t grp) { return lane_ok && grp < g_end && row_of(grp) < P.D; };
+  uint32_t st_nx = 0;
+  double d_nx = 0.0;
+  int64_t grp = p * ngroups / nblk;
+  if (use_h && row_valid(grp)) {
+    const int64_t r0 = row_of(grp);
+    st_nx = nt ? __builtin_nontemporal_load(states + r0) : states[r0];
+    d_nx = nt ? __builtin_nontemporal_load(diag + r0) : diag[r0];
+  }
+  for (; grp < g_end; ++grp) {
+    const int64_t row = row_of(grp);
+    const uint32_t st = st_nx;
+    const double drow = d_nx;
+    if (use_h && row_valid(grp + 1)) {
+      const int64_t r1 = row_of(grp + 1);
+      st_nx = nt ? __builtin_nontemporal_load(states + r1) : states[r1];
+      d_nx = nt ? __builtin_nontemporal_load(diag + r1) : diag[r1];
+    }
     if (!lane_ok || row >= P.D) continue;  // whole row groups drop out together
     vt h = (vt)(T)0;
-    if (flags & kHasH) {
-      const bool nt = flags & kStreamNT;
-      const uint32_t st = nt ? __builtin_nontemporal_load(states + row) : states[row];
-      const double drow = nt ? __builtin_nontemporal_load(diag + row) : diag[row];
+    if (use_h) {
       const uint32_t r32 = (uint32_t)row;
-      const T* __restrict__ B2c = B2 + cofs;
-      const uint32_t rs32 = (uint32_t)lay.rs;  // D*rs < 2^32 is checked on the host
       vt tot = (vt)(T)0;
       for (int b0 = 0; b0 < P.nb; b0 += NBAT) {
         uint32_t rk[NBAT];
@@ -208,7 +230,10 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
 #pragma unroll
         for (int j = 0; j < NBAT; ++j) {
           x[j] = (vt)(T)0;
-          if (rk[j] != r32) x[j] = vload<T, V>(B2c + rk[j] * rs32);
+          // 32-bit byte offset from the block base (D*rs*sizeof(T) < 2^32 is
+          // checked on the host)
+          const uint32_t o = (rsh >= 0 ? (rk[j] << rsh) : rk[j] * rsb) + cofs_b;
+          if (rk[j] != r32) x[j] = *reinterpret_cast<const vt*>(b2b + o);
         }
 #pragma unroll
         for (int j = 0; j < NBAT; ++j) {
@@ -217,7 +242,8 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
         }
       }
       const T d = (T)((drow - af.shift) * af.scale);
-      h = d * vload<T, V>(B2c + r32 * rs32) + tot;
+      const uint32_t o = (rsh >= 0 ? (r32 << rsh) : r32 * rsb) + cofs_b;
+      h = d * *reinterpret_cast<const vt*>(b2b + o) + tot;
     }
     step_epilogue<T, V>(flags, row, lay.rs, cofs, out_col0, cx, ch, cb, h, X, B3, OUT, out_ld, Pr,
                         dacc);
@@ -231,15 +257,20 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
 // ------------------------------------------------------ Quantum Sun kernel
 template <typename T, int V>
 __global__ __launch_bounds__(256) void qsun_step_kernel(
-    const QsunParams P, const Affine af, const int flags, const T cx, const T ch, const T cb,
-    const T* __restrict__ X, const T* __restrict__ B2, const T* __restrict__ B3,
-    void* __restrict__ OUT, const Layout lay, const int64_t out_ld, double* __restrict__ dot_part,
-    const T* __restrict__ Pr, const int64_t ngroups) {
+    const QsunParams P, const double* __restrict__ qdiag, const int qlb, const Affine af,
+    const int flags, const T cx, const T ch, const T cb, const T* __restrict__ X,
+    const T* __restrict__ B2, const T* __restrict__ B3, void* __restrict__ OUT, const Layout lay,
+    const int64_t out_ld, double* __restrict__ dot_part, const T* __restrict__ Pr,
+    const int64_t ngroups) {
   typedef typename VT<T, V>::t vt;
   const int dg = 1 << P.Lg;
-  __shared__ T sG[(1 << kMaxGrain) * (1 << kMaxGrain)];
+  __shared__ T sG[kMaxDg * kMaxDg];
+  __shared__ T sC[kMaxLoc];
+  __shared__ uint32_t sF[kMaxLoc];
   __shared__ double red[4];
-  for (int i = threadIdx.x; i < dg * dg; i += blockDim.x) sG[i] = (T)(P.G[i] * af.scale);
+  const uint32_t rsb = (uint32_t)(lay.rs * sizeof(T));
+  const int rsh = (rsb & (rsb - 1u)) == 0 ? __ffs(rsb) - 1 : -1;
+  qsun_stage<T>(P.Lloc, dg, P.flip, P.coup, P.G, af.scale, rsh, sF, sC, sG);
   __syncthreads();
 
   const int rloc = threadIdx.x / lay.G;
@@ -247,47 +278,34 @@ __global__ __launch_bounds__(256) void qsun_step_kernel(
   const bool lane_ok = rloc < lay.rows_per_block;
   const int64_t cofs = (int64_t)blockIdx.y * lay.cstride + (int64_t)g * V;
   const int out_col0 = (int)blockIdx.y + g * V;
+  QsunRow q;
+  // block-uniform column offset in the 64-bit base, the lane's own in 32 bits
+  q.b2 = reinterpret_cast<const char*>(B2 + (int64_t)blockIdx.y * lay.cstride);
+  q.cofs_b = (uint32_t)((int64_t)g * V * sizeof(T));
+  q.rsb = rsb;
+  q.rsh = rsh;
+  q.dg = dg;
+  q.nloc = P.Lloc;
+  q.lb = qlb;
+  q.dlo = qdiag;
+  q.dhi = qdiag + ((int64_t)1 << qlb);
+  q.scale = af.scale;
+  q.shift = af.shift;
+  q.sF = sF;
+  q.sC = sC;
+  const bool fixed_a = lay.rows_per_block % dg == 0;
+  T wg[kMaxDg];
+  qsun_lane_weights<T>(sG, dg, rloc & (dg - 1), wg);
   double dacc = 0.0;
 
-  const bool remap = lay.xcd && (int64_t)gridDim.x == ngroups;
-  for (int64_t it = blockIdx.x; it < ngroups; it += gridDim.x) {
-    const int64_t grp = remap ? xcd_block(it, ngroups) : it;
+  const int64_t nblk = gridDim.x;
+  const int64_t p = lay.xcd ? xcd_block(blockIdx.x, nblk) : (int64_t)blockIdx.x;
+  const int64_t g_end = (p + 1) * ngroups / nblk;
+  for (int64_t grp = p * ngroups / nblk; grp < g_end; ++grp) {
     const int64_t row = grp * lay.rows_per_block + rloc;
     if (!lane_ok || row >= P.D) continue;
     vt h = (vt)(T)0;
-    if (flags & kHasH) {
-      const uint32_t r = (uint32_t)row;
-      const int a = (int)(r & (uint32_t)(dg - 1));
-      const uint32_t base = r & ~(uint32_t)(dg - 1);
-      const T* __restrict__ B2c = B2 + cofs;
-      const uint32_t rs32 = (uint32_t)lay.rs;
-      // diagonal in the reference order: outer fields, then the grain diagonal
-      double dd = 0.0;
-      for (int j = 0; j < P.Lloc; ++j) dd += P.h[j] * (((r >> (P.Lg + j)) & 1u) ? 0.5 : -0.5);
-      dd += P.G[a * dg + a];
-      const T d = (T)((dd - af.shift) * af.scale);
-      // grain block: the whole dg-row group, the diagonal element weighted by d
-      vt acc = (vt)(T)0;
-      for (int b = 0; b < dg; ++b) {
-        const T wb = (b == a) ? d : sG[a * dg + b];
-        acc += wb * vload<T, V>(B2c + (base | (uint32_t)b) * rs32);
-      }
-      // outer-spin flips (always present in the full space): batched gathers
-      for (int j0 = 0; j0 < P.Lloc; j0 += kGatherBatch) {
-        vt x[kGatherBatch];
-        T w[kGatherBatch];
-#pragma unroll
-        for (int u = 0; u < kGatherBatch; ++u) {
-          const int j = j0 + u;
-          const bool ok = j < P.Lloc;  // uniform
-          w[u] = ok ? (T)(P.coup[j] * af.scale) : (T)0;
-          x[u] = vload<T, V>(B2c + (ok ? (r ^ P.flip[j]) : r) * rs32);
-        }
-#pragma unroll
-        for (int u = 0; u < kGatherBatch; ++u) acc += w[u] * x[u];
-      }
-      h = acc;
-    }
+    if (flags & kHasH) h = qsun_hrow<T, V>(q, (uint32_t)row, fixed_a, wg, sG);
     step_epilogue<T, V>(flags, row, lay.rs, cofs, out_col0, cx, ch, cb, h, X, B3, OUT, out_ld, Pr,
                         dacc);
   }
@@ -306,7 +324,7 @@ static bool valid_vec(int dtype, int s, int V) {
   return s / V <= 256;
 }
 
-Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt, int pb) {
+Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt, int pb, int wide) {
   Layout l{};
   if (V == 0 || !valid_vec(dtype, s, V)) {
     if (dtype == PF_F64)
@@ -322,6 +340,7 @@ Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt, int pb) {
   l.xcd = xcd;
   l.nt = nt;
   l.pblocks = pb;
+  l.wide = wide;
   return l;
 }
 
@@ -340,8 +359,9 @@ Layout tuned_layout(pf_op* op, int dtype, int s) {
   if (op->force.xcd >= 0) c.xcd = op->force.xcd;
   if (op->force.nt >= 0) c.nt = op->force.nt;
   if (op->force.pb >= 0) c.pb = op->force.pb;
+  if (op->force.wide >= 0) c.wide = op->force.wide;
   return interleaved_layout(dtype, s, c.V, c.xcd < 0 ? 1 : c.xcd, c.nt < 0 ? 1 : c.nt,
-                            c.pb < 0 ? default_pblocks(op) : c.pb);
+                            c.pb < 0 ? default_pblocks(op) : c.pb, c.wide < 0 ? 0 : c.wide);
 }
 
 Layout colmajor_layout(int64_t ld) {
@@ -372,8 +392,8 @@ static void launch_typed(pf_op* op, int flags, const Layout& lay, int ncols_grid
                          double cx, double ch, double cb, const void* X, const void* B2,
                          const void* B3, void* out, int64_t out_ld, hipStream_t st,
                          double* dot_part, const void* P, int dot_blocks) {
-  PF_REQUIRE((uint64_t)op->D * (uint64_t)lay.rs < ((uint64_t)1 << 32),
-             "block too large for 32-bit row offsets");
+  PF_REQUIRE((uint64_t)op->D * (uint64_t)lay.rs * sizeof(T) < ((uint64_t)1 << 32),
+             "block too large for 32-bit byte offsets");
   const int64_t ngroups = (op->D + lay.rows_per_block - 1) / lay.rows_per_block;
   const int64_t gx = step_grid_x(op, lay, flags, dot_blocks);
   PF_REQUIRE(gx < (int64_t)1 << 31, "grid too large");
@@ -388,28 +408,31 @@ static void launch_typed(pf_op* op, int flags, const Layout& lay, int ncols_grid
     const uint4* rt = reinterpret_cast<const uint4*>(op->rtab);
     const int rvec = (int)(op->rtab_bytes / 16);
     const size_t lds = tbl ? op->rtab_bytes : 0;
-#define PF_CHAIN_LAUNCH(GMV, TB)                                                               \
-  hipLaunchKernelGGL((chain_step_kernel<T, V, GMV, TB>), grid, block, lds, st, op->chain,      \
+#define PF_CHAIN_LAUNCH(GMV, TB, NW)                                                           \
+  hipLaunchKernelGGL((chain_step_kernel<T, V, GMV, TB, NW>), grid, block, lds, st, op->chain,  \
                      op->states, op->binom, rt, rvec, op->diag, af, flags, (T)cx, (T)ch, (T)cb, \
                      Xt, B2t, B3t, out, lay, out_ld, dot_part, Pt, ngroups)
-#define PF_CHAIN_G(TB)                      \
-  switch (lay.G) {                          \
-    case 1: PF_CHAIN_LAUNCH(1, TB); break;  \
-    case 2: PF_CHAIN_LAUNCH(2, TB); break;  \
-    case 4: PF_CHAIN_LAUNCH(4, TB); break;  \
-    case 8: PF_CHAIN_LAUNCH(8, TB); break;  \
-    default: PF_CHAIN_LAUNCH(0, TB); break; \
+#define PF_CHAIN_G(TB, NW)                      \
+  switch (lay.G) {                              \
+    case 1: PF_CHAIN_LAUNCH(1, TB, NW); break;  \
+    case 2: PF_CHAIN_LAUNCH(2, TB, NW); break;  \
+    case 4: PF_CHAIN_LAUNCH(4, TB, NW); break;  \
+    case 8: PF_CHAIN_LAUNCH(8, TB, NW); break;  \
+    default: PF_CHAIN_LAUNCH(0, TB, NW); break; \
   }
-    if (tbl) {
-      PF_CHAIN_G(true);
+    if (tbl && lay.wide) {
+      PF_CHAIN_G(true, 2);
+    } else if (tbl) {
+      PF_CHAIN_G(true, 1);
     } else {
-      PF_CHAIN_G(false);
+      PF_CHAIN_G(false, 1);
     }
 #undef PF_CHAIN_G
 #undef PF_CHAIN_LAUNCH
   } else {
-    hipLaunchKernelGGL((qsun_step_kernel<T, V>), grid, block, 0, st, op->qsun, af, flags, (T)cx,
-                       (T)ch, (T)cb, Xt, B2t, B3t, out, lay, out_ld, dot_part, Pt, ngroups);
+    hipLaunchKernelGGL((qsun_step_kernel<T, V>), grid, block, 0, st, op->qsun, op->qdiag, op->qlb,
+                       af, flags, (T)cx, (T)ch, (T)cb, Xt, B2t, B3t, out, lay, out_ld, dot_part, Pt,
+                       ngroups);
   }
   PF_HIP(hipGetLastError());
 }
@@ -821,6 +844,34 @@ extern "C" int pf_op_create_qsun(pf_handle* h, int32_t L_grain, int32_t L_loc, d
     // row-major copy of the (column-major) grain matrix: G[a*dg + b] = H[a, b]
     for (int a = 0; a < dg; ++a)
       for (int b = 0; b < dg; ++b) P.G[a * dg + b] = H_grain[a + b * dg];
+    // Diagonal split: d(r) = Dlo[r mod 2^lb] + Dhi[r >> lb], Dlo carrying the
+    // grain diagonal and the z-fields of the outer spins below bit lb.
+    const int lb = L_grain + (L_loc + 1) / 2;
+    const int hb = L_grain + L_loc - lb;
+    std::vector<double> tab(((size_t)1 << lb) + ((size_t)1 << hb));
+    for (uint32_t t = 0; t < (1u << lb); ++t) {
+      double v = 0.0;
+      for (int j = 0; L_grain + j < lb; ++j) v += fields[j] * (((t >> (L_grain + j)) & 1u) ? 0.5 : -0.5);
+      tab[t] = v + P.G[(t & (dg - 1)) * (dg + 1)];
+    }
+    for (uint32_t u = 0; u < (1u << hb); ++u) {
+      double v = 0.0;
+      for (int j = lb - L_grain; j < L_loc; ++j)
+        v += fields[j] * (((u >> (L_grain + j - lb)) & 1u) ? 0.5 : -0.5);
+      tab[((size_t)1 << lb) + u] = v;
+    }
+    op->qlb = lb;
+    if (hipMalloc(&op->qdiag, tab.size() * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
+      delete op;
+      throw Error{PF_ERR_NOMEM, "qsun: diagonal table allocation failed"};
+    }
+    if (hipMemcpy(op->qdiag, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) !=
+        hipSuccess) {
+      (void)hipFree(op->qdiag);
+      delete op;
+      throw Error{PF_ERR_HIP, "qsun: diagonal table upload failed"};
+    }
     *out = op;
   });
 }
@@ -831,6 +882,7 @@ extern "C" int pf_op_destroy(pf_op* op) {
   if (op->binom) (void)hipFree(op->binom);
   if (op->diag) (void)hipFree(op->diag);
   if (op->rtab) (void)hipFree(op->rtab);
+  if (op->qdiag) (void)hipFree(op->qdiag);
   op->fbuf.release();
   op->kpart.release();
   delete op;
@@ -1112,6 +1164,9 @@ extern "C" int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value) {
     } else if (key == PF_TUNE_NONTEMPORAL) {
       PF_REQUIRE(value >= -1 && value <= 1, "tuning: non-temporal is -1, 0 or 1");
       op->force.nt = value;
+    } else if (key == PF_TUNE_WIDE) {
+      PF_REQUIRE(value >= -1 && value <= 1, "tuning: wide gather batches is -1, 0 or 1");
+      op->force.wide = value;
     } else if (key == PF_TUNE_PBLOCKS) {
       PF_REQUIRE(value >= -1 && value <= 64, "tuning: persistent blocks per CU is -1..64");
       op->force.pb = value;
@@ -1156,21 +1211,26 @@ extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nste
     const int pbs[3] = {default_pblocks(op), 4, 0};
     for (int vi = 0; vi < 6; ++vi) {
       if (!valid_vec(dtype, ncols, vs[vi])) continue;
+      const int nwide = (op->kind == kChain && op->chain.tsplit > 0) ? 2 : 1;
       for (int pi = 0; pi < 3; ++pi) {
         if (pi > 0 && pbs[pi] == pbs[0]) continue;
         for (int x = 1; x >= 0; --x) {
-          TuneCfg c;
-          c.V = vs[vi];
-          c.xcd = x;
-          c.nt = 1;
-          c.pb = pbs[pi];
-          cands.push_back({c, 1e300});
+          for (int w = 0; w < nwide; ++w) {
+            TuneCfg c;
+            c.V = vs[vi];
+            c.xcd = x;
+            c.nt = 1;
+            c.pb = pbs[pi];
+            c.wide = w;
+            cands.push_back({c, 1e300});
+          }
         }
       }
     }
     for (int round = 0; round < 3; ++round) {
       for (auto& cd : cands) {
-        const Layout lay = interleaved_layout(dtype, ncols, cd.c.V, cd.c.xcd, cd.c.nt, cd.c.pb);
+        const Layout lay =
+            interleaved_layout(dtype, ncols, cd.c.V, cd.c.xcd, cd.c.nt, cd.c.pb, cd.c.wide);
         for (int w = 0; w < 2; ++w)
           launch_step(op, dtype, kHasX | kHasH | kHasB3, lay, ncols, af, 0.5, 2.0, -1.0, b[0],
                       b[1], b[2], b[3], 0, st);

@@ -139,6 +139,7 @@ struct Layout {
   int xcd;         // 1: map consecutive row groups to the same XCD (L2 locality)
   int nt;          // 1: non-temporal hints on streamed loads/stores
   int pblocks;     // persistent grid: blocks per CU (0: one block per row group)
+  int wide;        // 1: twice the neighbour gathers in flight per batch (spin chains)
 };
 
 // Launch configuration of the row-interleaved step kernel for one (dtype, s).
@@ -147,6 +148,7 @@ struct TuneCfg {
   int xcd = -1; // XCD-contiguous row mapping (-1 = default)
   int nt = -1;  // non-temporal streamed accesses (-1 = default)
   int pb = -1;  // persistent blocks per CU (-1 = default)
+  int wide = -1;  // wide gather batches (-1 = default)
 };
 
 }  // namespace pf
@@ -169,6 +171,8 @@ struct pf_op {
   uint32_t* states = nullptr;  // spin chain: ascending basis states
   uint32_t* binom = nullptr;   // spin chain: C(p, t), p < 32, t < kRankT
   double* diag = nullptr;      // spin chain: unscaled diagonal, reference summation order
+  double* qdiag = nullptr;     // Quantum Sun: Dlo[2^qlb] then Dhi[2^(Lg+Lloc-qlb)] (diagonal)
+  int qlb = 0;                 // Quantum Sun: low bits of the diagonal split
   void* rtab = nullptr;        // spin chain: u16 TA[2^m] then u32 TB[2^(L-m)] (rank tables)
   size_t rtab_bytes = 0;       // bytes of rtab (dynamic LDS of the step kernel)
   pf::DevBuf fbuf;             // filter / KPM row-interleaved work blocks
@@ -194,7 +198,8 @@ void launch_step(pf_op* op, int dtype, int flags, const Layout& lay, int ncols, 
                  void* out, int64_t out_ld, hipStream_t st, double* dot_part = nullptr,
                  const void* P = nullptr, int dot_blocks = 0);
 
-Layout interleaved_layout(int dtype, int s, int V = 0, int xcd = 1, int nt = 1, int pb = 0);
+Layout interleaved_layout(int dtype, int s, int V = 0, int xcd = 1, int nt = 1, int pb = 0,
+                          int wide = 0);
 Layout tuned_layout(pf_op* op, int dtype, int s);
 Layout colmajor_layout(int64_t ld);
 

@@ -108,9 +108,10 @@ def test_long_chain_without_rank_tables(dev):
     assert _rel(dev.to_host(op.filter(c, dev.to_device(X))), oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)) < 1e-12
 
 
-@pytest.mark.parametrize("pblocks", [0, 1, 8])
-def test_filter_persistent_grid_settings_agree(dev, pblocks):
-    """The persistent-grid knob changes only the block -> row mapping."""
+@pytest.mark.parametrize("pblocks,wide,vec", [(0, 0, 2), (1, 1, 1), (8, 1, 4), (8, 0, 8)])
+def test_filter_persistent_grid_settings_agree(dev, pblocks, wide, vec):
+    """The persistent-grid, gather-batch and vector-width knobs change only
+    how rows and neighbours are scheduled, not the result."""
     op, H = _chain(dev, *CHAINS[2][1:])
     w = np.linalg.eigvalsh(H.toarray())
     op.set_rescale(w[0], w[-1])
@@ -118,7 +119,7 @@ def test_filter_persistent_grid_settings_agree(dev, pblocks):
     Ht = (H - b * sp.identity(H.shape[0])) / a
     c = oc.filter_coefficients(-0.2, 150)
     X = np.random.default_rng(7).standard_normal((H.shape[0], 8))
-    op.set_tuning(pblocks=pblocks)
+    op.set_tuning(pblocks=pblocks, wide=wide, vec=vec)
     Y = dev.to_host(op.filter(c, dev.to_device(X)))
     assert _rel(Y, oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)) < 1e-12
 

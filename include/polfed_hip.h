@@ -278,7 +278,7 @@ enum {
   PF_BATCH_GRAPH = 1,          /* 0: stream launches, 1: graph replays (default) */
   PF_BATCH_GRAPH_STEPS = 2,    /* step nodes per captured graph (default 96)      */
   PF_BATCH_VEC = 3,            /* columns per lane: 1/2/4/8 dividing ncols        */
-  PF_BATCH_ROWS_PER_LANE = 4,  /* rows per lane: 1/2/4                            */
+  PF_BATCH_ROWS_PER_LANE = 4,  /* rows per lane: 1..64                            */
   PF_BATCH_XCD = 5             /* 1: XCD-contiguous block mapping (default)       */
 };
 /* Launch settings of the batch.  No reference counterpart. */

@@ -68,6 +68,7 @@ struct BatchArgs {
   int s;     // columns per instance
   int G;     // lanes per row
   int rpb;   // rows per block pass (256 / G)
+  int rpt;   // block passes (rows per lane)
   int xcd;   // XCD-contiguous block mapping
 };
 
@@ -83,17 +84,18 @@ __device__ __forceinline__ void poly_ab_dev(int type, int k, double& alpha, doub
 // Buffer slot written at local step g (rotation (3,1,2) of the recurrence).
 __device__ __forceinline__ int rot_slot(int g) { return (3 - g % 3) % 3; }
 
-template <typename T, int V, int RPT>
+template <typename T, int V>
 __device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& I,
                                            const BatchDyn& dy, int k, int64_t lb) {
   typedef typename VT<T, V>::t vt;
   __shared__ T sG[kMaxDg * kMaxDg];
-  __shared__ T sC[kMaxLoc];
-  __shared__ uint32_t sF[kMaxLoc];
+  __shared__ T sC[kMaxLocPad];
+  __shared__ uint32_t sF[kMaxLocPad];
   const int dg = 1 << I.Lg;
   const uint32_t rsb = (uint32_t)(A.s * sizeof(T));
   const int rsh = (rsb & (rsb - 1u)) == 0 ? __ffs(rsb) - 1 : -1;
-  qsun_stage<T>(I.Lloc, dg, I.flip, I.coup, I.G, I.scale, rsh, sF, sC, sG);
+  const int npad = qsun_pad(I.Lloc, qsun_flip_batch<T, V>());
+  qsun_stage<T>(I.Lloc, npad, dg, I.flip, I.coup, I.G, I.scale, rsh, sF, sC, sG);
   __syncthreads();
 
   const int g = dy.K - k;  // local step, >= 0
@@ -134,7 +136,7 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& 
   q.rsb = rsb;
   q.rsh = rsh;
   q.dg = dg;
-  q.nloc = I.Lloc;
+  q.nloc = npad;
   q.lb = I.lb;
   q.dlo = I.qdiag;
   q.dhi = I.qdiag + ((int64_t)1 << I.lb);
@@ -142,19 +144,17 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& 
   q.shift = I.shift;
   q.sF = sF;
   q.sC = sC;
-  const bool fixed_a = A.rpb % dg == 0;
-  T wg[kMaxDg];
-  qsun_lane_weights<T>(sG, dg, rloc & (dg - 1), wg);
-#pragma unroll
-  for (int u = 0; u < RPT; ++u) {
-    const int64_t row = (lb * RPT + u) * A.rpb + rloc;
+  for (int u = 0; u < A.rpt; ++u) {
+    const int64_t row = (lb * A.rpt + u) * A.rpb + rloc;
     if (row >= I.D) break;
-    vt hv = (vt)(T)0;
-    if (hasH) hv = qsun_hrow<T, V>(q, (uint32_t)row, fixed_a, wg, sG);
     const int64_t o = row * A.s + cofs;
-    vt out = cx * vload<T, V>(X + o);
+    const vt xv = vload<T, V>(X + o);
+    const vt b3v = hasB3 ? vload<T, V>(B3 + o) : (vt)(T)0;
+    vt hv = (vt)(T)0;
+    if (hasH) hv = qsun_hrow_dg<T, V>(q, (uint32_t)row, sG);
+    vt out = cx * xv;
     if (hasH) out += ch * hv;
-    if (hasB3) out += cb * vload<T, V>(B3 + o);
+    if (hasB3) out += cb * b3v;
     if (k >= 1) {
       vstore<T, V>(OUT + o, out);
     } else {
@@ -165,9 +165,9 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& 
   }
 }
 
-// One Clenshaw step of every active instance.  kexpl >= 0: the step index;
-// kexpl < 0: k = kstate[0] - j (graph nodes).
-template <int V, int RPT>
+// One Clenshaw step of every active instance whose arithmetic is T.
+// kexpl >= 0: the step index; kexpl < 0: k = kstate[0] - j (graph nodes).
+template <typename T, int V>
 __global__ __launch_bounds__(256) void qsun_batch_step_kernel(const BatchArgs A, int kexpl,
                                                               int j) {
   const int64_t nb = gridDim.x;
@@ -181,15 +181,15 @@ __global__ __launch_bounds__(256) void qsun_batch_step_kernel(const BatchArgs A,
   }
   const BatchDyn dy = A.dyn[lo];
   const int k = kexpl >= 0 ? kexpl : A.kstate[0] - j;
-  if (k < 0 || dy.K < 0 || k > dy.K) return;  // uniform over the block
+  constexpr int kDt = sizeof(T) == 8 ? PF_F64 : PF_F32;
+  if (k < 0 || dy.K < 0 || k > dy.K || dy.dtype != kDt) return;  // uniform over the block
   const BatchInst& I = A.insts[lo];
   const int64_t lb = b - A.blk0[lo];
-  if (dy.dtype == PF_F64) {
-    batch_body<double, V, RPT>(A, I, dy, k, lb);
-  } else {
-    batch_body<float, V, RPT>(A, I, dy, k, lb);
-  }
+  batch_body<T, V>(A, I, dy, k, lb);
 }
+
+// dtype bit set of an application: bit 0 FP64 instances, bit 1 FP32 ones
+constexpr int kMaskF64 = 1, kMaskF32 = 2;
 
 __global__ void batch_kset_kernel(int* kstate, int value) {
   if (threadIdx.x == 0 && blockIdx.x == 0) kstate[0] = value;
@@ -229,16 +229,18 @@ struct pf_batch {
   bool dyn_pending = false;
   // graph state
   hipStream_t cap = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};      // by dtype mask
+  hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
 
   int G() const { return s / V; }
   int rpb() const { return 256 / G(); }
   void drop_graph() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
+    for (int m = 0; m < 4; ++m) {
+      if (exec[m]) (void)hipGraphExecDestroy(exec[m]);
+      if (graph[m]) (void)hipGraphDestroy(graph[m]);
+      exec[m] = nullptr;
+      graph[m] = nullptr;
+    }
   }
   // Stage per-application records and copy them to the device on st.
   void upload_dyn(const std::vector<BatchDyn>& dy, hipStream_t st) {
@@ -317,40 +319,40 @@ struct pf_batch {
     A.s = s;
     A.G = G();
     A.rpb = rpb();
+    A.rpt = rpt;
     A.xcd = xcd;
     return A;
   }
-  void launch(const BatchArgs& A, int64_t grid, int kexpl, int j, hipStream_t st) {
-    if (grid <= 0) return;
-#define PF_BATCH_LAUNCH(VV, RR)                                                               \
-  hipLaunchKernelGGL((qsun_batch_step_kernel<VV, RR>), dim3((unsigned)grid), dim3(256), 0, st, \
-                     A, kexpl, j)
-#define PF_BATCH_RPT(VV)                       \
-  switch (rpt) {                               \
-    case 2: PF_BATCH_LAUNCH(VV, 2); break;     \
-    case 4: PF_BATCH_LAUNCH(VV, 4); break;     \
-    default: PF_BATCH_LAUNCH(VV, 1); break;    \
-  }
+  template <typename T>
+  void launch_t(const BatchArgs& A, int64_t grid, int kexpl, int j, hipStream_t st) {
+#define PF_BATCH_LAUNCH(VV)                                                          \
+  hipLaunchKernelGGL((qsun_batch_step_kernel<T, VV>), dim3((unsigned)grid), dim3(256), \
+                     0, st, A, kexpl, j)
     switch (V) {
-      case 2: PF_BATCH_RPT(2); break;
-      case 4: PF_BATCH_RPT(4); break;
-      case 8: PF_BATCH_RPT(8); break;
-      default: PF_BATCH_RPT(1); break;
+      case 2: PF_BATCH_LAUNCH(2); break;
+      case 4: PF_BATCH_LAUNCH(4); break;
+      case 8: PF_BATCH_LAUNCH(8); break;
+      default: PF_BATCH_LAUNCH(1); break;
     }
-#undef PF_BATCH_RPT
 #undef PF_BATCH_LAUNCH
+  }
+  // one step launch per arithmetic present in the application (mask)
+  void launch(const BatchArgs& A, int64_t grid, int kexpl, int j, hipStream_t st, int mask) {
+    if (grid <= 0) return;
+    if (mask & kMaskF64) launch_t<double>(A, grid, kexpl, j, st);
+    if (mask & kMaskF32) launch_t<float>(A, grid, kexpl, j, st);
     PF_HIP(hipGetLastError());
   }
   // timing: capture with constant coefficients (no coefficient reads), for
   // the autotuner's out-of-range step indices
-  void ensure_graph(bool timing = false) {
-    if (exec) return;
+  void ensure_graph(int mask, bool timing = false) {
+    if (exec[mask]) return;
     if (!cap) PF_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
     BatchArgs A = args();
     if (timing) A.coefs = nullptr;
     PF_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
     try {
-      for (int j = 0; j < gsteps; ++j) launch(A, blk0[n], -1, j, cap);
+      for (int j = 0; j < gsteps; ++j) launch(A, blk0[n], -1, j, cap, mask);
       hipLaunchKernelGGL(batch_kdec_kernel, dim3(1), dim3(64), 0, cap,
                          reinterpret_cast<int*>(kstate.p), gsteps);
       PF_HIP(hipGetLastError());
@@ -360,26 +362,26 @@ struct pf_batch {
       if (g) (void)hipGraphDestroy(g);
       throw;
     }
-    PF_HIP(hipStreamEndCapture(cap, &graph));
-    PF_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    PF_HIP(hipStreamEndCapture(cap, &graph[mask]));
+    PF_HIP(hipGraphInstantiate(&exec[mask], graph[mask], nullptr, nullptr, 0));
   }
   // Run steps kmax .. 0 of the current application on stream st.
-  void run_steps(int kmax, hipStream_t st) {
+  void run_steps(int kmax, hipStream_t st, int mask) {
     const BatchArgs A = args();
     if (use_graph) {
-      ensure_graph();
+      ensure_graph(mask);
       hipLaunchKernelGGL(batch_kset_kernel, dim3(1), dim3(64), 0, st,
                          reinterpret_cast<int*>(kstate.p), kmax);
       PF_HIP(hipGetLastError());
       const int reps = (kmax + 1 + gsteps - 1) / gsteps;
-      for (int r = 0; r < reps; ++r) PF_HIP(hipGraphLaunch(exec, st));
+      for (int r = 0; r < reps; ++r) PF_HIP(hipGraphLaunch(exec[mask], st));
     } else {
       // active at step k: launch positions with K >= k, a prefix that grows
       // as k falls (K is non-increasing in launch order)
       int nact = 0;
       for (int k = kmax; k >= 0; --k) {
         while (nact < n && K[nact] >= k) ++nact;
-        launch(A, blk0[nact], k, 0, st);
+        launch(A, blk0[nact], k, 0, st, mask);
       }
     }
   }
@@ -489,6 +491,7 @@ extern "C" int pf_batch_filter(pf_batch* b, const int32_t* active, const int32_t
     const int n = b->n;
     std::vector<BatchDyn> dy(n);
     int kmax = -1;
+    int mask = 0;
     for (int i = 0; i < n; ++i) {
       const int p = b->pos[i];
       BatchDyn& d = dy[p];
@@ -506,6 +509,7 @@ extern "C" int pf_batch_filter(pf_batch* b, const int32_t* active, const int32_t
       d.Y = Y[i];
       d.ldy = ldy[i];
       kmax = std::max(kmax, d.K);
+      mask |= dtypes[i] == PF_F64 ? kMaskF64 : kMaskF32;
     }
     if (kmax < 0) return;
     b->upload_dyn(dy, st);
@@ -516,7 +520,7 @@ extern "C" int pf_batch_filter(pf_batch* b, const int32_t* active, const int32_t
       pack_block(dtypes[i], X[i], ldx[i], base + b->row0[p] * (int64_t)b->s * 8, b->ops[i]->D,
                  b->s, nullptr, st);
     }
-    b->run_steps(kmax, st);
+    b->run_steps(kmax, st, mask);
   });
 }
 
@@ -539,7 +543,7 @@ extern "C" int pf_batch_set_tuning(pf_batch* b, int32_t key, int32_t value) {
         b->layout_blocks();
         break;
       case PF_BATCH_ROWS_PER_LANE:
-        PF_REQUIRE(value == 1 || value == 2 || value == 4, "batch: rows per lane 1/2/4");
+        PF_REQUIRE(value >= 1 && value <= 64, "batch: 1 <= rows per lane <= 64");
         b->rpt = value;
         b->layout_blocks();
         break;
@@ -561,6 +565,7 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
     PF_REQUIRE(nsteps >= 3 && nsteps <= 100000, "batch: 3 <= nsteps <= 100000");
     hipStream_t st = S(stream);
     const int n = b->n;
+    const int tmask = dtype == PF_F64 ? kMaskF64 : kMaskF32;
     // timing mode: every instance active at every step, constant coefficients
     std::vector<BatchDyn> dy(n);
     for (auto& d : dy) {
@@ -581,7 +586,7 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
     std::vector<Cand> cands;
     for (int V : {1, 2, 4, 8})
       if (batch_vec_ok(b->s, V) && !(dtype == PF_F64 && V > 8))
-        for (int rpt : {1, 2, 4})
+        for (int rpt : {1, 4, 16, 64})
           for (int x : {1, 0}) cands.push_back({V, rpt, x, 1e300});
     const int saveV = b->V, saveR = b->rpt, saveX = b->xcd;
     const bool had_filter = b->have_filter;
@@ -595,9 +600,9 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
         A.coefs = nullptr;
         const int64_t grid = b->blk0[n];
         const int kbig = 1 << 28;
-        for (int w = 0; w < 2; ++w) b->launch(A, grid, kbig, 0, st);
+        for (int w = 0; w < 2; ++w) b->launch(A, grid, kbig, 0, st, tmask);
         PF_HIP(hipEventRecord(e0, st));
-        for (int k = 0; k < nsteps; ++k) b->launch(A, grid, kbig - k, 0, st);
+        for (int k = 0; k < nsteps; ++k) b->launch(A, grid, kbig - k, 0, st, tmask);
         PF_HIP(hipEventRecord(e1, st));
         PF_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -621,15 +626,15 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
       const int saveG = b->gsteps;
       b->gsteps = std::min(saveG, nsteps);
       b->drop_graph();
-      b->ensure_graph(true);
+      b->ensure_graph(tmask, true);
       double g_us = 1e300, s_us = 1e300;
       for (int round = 0; round < 3; ++round) {
         hipLaunchKernelGGL(batch_kset_kernel, dim3(1), dim3(64), 0, st,
                            reinterpret_cast<int*>(b->kstate.p), 1 << 28);
         PF_HIP(hipGetLastError());
-        PF_HIP(hipGraphLaunch(b->exec, st));
+        PF_HIP(hipGraphLaunch(b->exec[tmask], st));
         PF_HIP(hipEventRecord(e0, st));
-        for (int r = 0; r < 2; ++r) PF_HIP(hipGraphLaunch(b->exec, st));
+        for (int r = 0; r < 2; ++r) PF_HIP(hipGraphLaunch(b->exec[tmask], st));
         PF_HIP(hipEventRecord(e1, st));
         PF_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -638,7 +643,8 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
         BatchArgs A = b->args();
         A.coefs = nullptr;
         PF_HIP(hipEventRecord(e0, st));
-        for (int k = 0; k < 2 * b->gsteps; ++k) b->launch(A, b->blk0[n], (1 << 27) - k, 0, st);
+        for (int k = 0; k < 2 * b->gsteps; ++k)
+          b->launch(A, b->blk0[n], (1 << 27) - k, 0, st, tmask);
         PF_HIP(hipEventRecord(e1, st));
         PF_HIP(hipEventSynchronize(e1));
         PF_HIP(hipEventElapsedTime(&ms, e0, e1));

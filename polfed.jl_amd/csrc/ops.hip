@@ -24,21 +24,34 @@
 
 namespace pf {
 
-// Shared epilogue: out = cx*X + ch*h + cb*b3, store, optional KPM dot.
+// Streamed inputs of one row (X and b3), loaded before the neighbour gathers
+// so that their latency overlaps them.
 template <typename T, int V>
-__device__ __forceinline__ void step_epilogue(int flags, int64_t row, int64_t rs, int64_t cofs,
-                                              int out_col0, T cx, T ch, T cb,
-                                              typename VT<T, V>::t h, const T* __restrict__ X,
-                                              const T* __restrict__ B3, void* __restrict__ OUT,
-                                              int64_t out_ld, const T* __restrict__ Pr,
-                                              double& dacc) {
+__device__ __forceinline__ void step_load(int flags, int64_t off, const T* __restrict__ X,
+                                          const T* __restrict__ B3, typename VT<T, V>::t& xv,
+                                          typename VT<T, V>::t& b3v) {
   typedef typename VT<T, V>::t vt;
-  const int64_t off = row * rs + cofs;
+  const bool nt = flags & kStreamNT;
+  xv = (vt)(T)0;
+  b3v = (vt)(T)0;
+  if (flags & kHasX) xv = nt ? vload_nt<T, V>(X + off) : vload<T, V>(X + off);
+  if (flags & kHasB3) b3v = nt ? vload_nt<T, V>(B3 + off) : vload<T, V>(B3 + off);
+}
+
+// out = cx*X + ch*h + cb*b3, store (interleaved T, or FP64 column-major with
+// kOutCM), optional KPM dot <P, out>.
+template <typename T, int V>
+__device__ __forceinline__ void step_store(int flags, int64_t row, int64_t off, int out_col0,
+                                           T cx, T ch, T cb, typename VT<T, V>::t h,
+                                           typename VT<T, V>::t xv, typename VT<T, V>::t b3v,
+                                           void* __restrict__ OUT, int64_t out_ld,
+                                           const T* __restrict__ Pr, double& dacc) {
+  typedef typename VT<T, V>::t vt;
   vt o = (vt)(T)0;
   const bool nt = flags & kStreamNT;
-  if (flags & kHasX) o = cx * (nt ? vload_nt<T, V>(X + off) : vload<T, V>(X + off));
+  if (flags & kHasX) o = cx * xv;
   if (flags & kHasH) o += ch * h;
-  if (flags & kHasB3) o += cb * (nt ? vload_nt<T, V>(B3 + off) : vload<T, V>(B3 + off));
+  if (flags & kHasB3) o += cb * b3v;
   if (flags & kOutCM) {
     double* Y = reinterpret_cast<double*>(OUT);
 #pragma unroll
@@ -219,6 +232,9 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
       d_nx = nt ? __builtin_nontemporal_load(diag + r1) : diag[r1];
     }
     if (!lane_ok || row >= P.D) continue;  // whole row groups drop out together
+    const int64_t off = row * lay.rs + cofs;
+    vt xv, b3v;
+    step_load<T, V>(flags, off, X, B3, xv, b3v);
     vt h = (vt)(T)0;
     if (use_h) {
       const uint32_t r32 = (uint32_t)row;
@@ -245,8 +261,7 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
       const uint32_t o = (rsh >= 0 ? (r32 << rsh) : r32 * rsb) + cofs_b;
       h = d * *reinterpret_cast<const vt*>(b2b + o) + tot;
     }
-    step_epilogue<T, V>(flags, row, lay.rs, cofs, out_col0, cx, ch, cb, h, X, B3, OUT, out_ld, Pr,
-                        dacc);
+    step_store<T, V>(flags, row, off, out_col0, cx, ch, cb, h, xv, b3v, OUT, out_ld, Pr, dacc);
   }
   if (flags & kDot) {
     const double t = block_sum_256(dacc, red);
@@ -265,12 +280,13 @@ __global__ __launch_bounds__(256) void qsun_step_kernel(
   typedef typename VT<T, V>::t vt;
   const int dg = 1 << P.Lg;
   __shared__ T sG[kMaxDg * kMaxDg];
-  __shared__ T sC[kMaxLoc];
-  __shared__ uint32_t sF[kMaxLoc];
+  __shared__ T sC[kMaxLocPad];
+  __shared__ uint32_t sF[kMaxLocPad];
   __shared__ double red[4];
   const uint32_t rsb = (uint32_t)(lay.rs * sizeof(T));
   const int rsh = (rsb & (rsb - 1u)) == 0 ? __ffs(rsb) - 1 : -1;
-  qsun_stage<T>(P.Lloc, dg, P.flip, P.coup, P.G, af.scale, rsh, sF, sC, sG);
+  const int npad = qsun_pad(P.Lloc, qsun_flip_batch<T, V>());
+  qsun_stage<T>(P.Lloc, npad, dg, P.flip, P.coup, P.G, af.scale, rsh, sF, sC, sG);
   __syncthreads();
 
   const int rloc = threadIdx.x / lay.G;
@@ -285,7 +301,7 @@ __global__ __launch_bounds__(256) void qsun_step_kernel(
   q.rsb = rsb;
   q.rsh = rsh;
   q.dg = dg;
-  q.nloc = P.Lloc;
+  q.nloc = npad;
   q.lb = qlb;
   q.dlo = qdiag;
   q.dhi = qdiag + ((int64_t)1 << qlb);
@@ -293,9 +309,6 @@ __global__ __launch_bounds__(256) void qsun_step_kernel(
   q.shift = af.shift;
   q.sF = sF;
   q.sC = sC;
-  const bool fixed_a = lay.rows_per_block % dg == 0;
-  T wg[kMaxDg];
-  qsun_lane_weights<T>(sG, dg, rloc & (dg - 1), wg);
   double dacc = 0.0;
 
   const int64_t nblk = gridDim.x;
@@ -304,10 +317,12 @@ __global__ __launch_bounds__(256) void qsun_step_kernel(
   for (int64_t grp = p * ngroups / nblk; grp < g_end; ++grp) {
     const int64_t row = grp * lay.rows_per_block + rloc;
     if (!lane_ok || row >= P.D) continue;
+    const int64_t off = row * lay.rs + cofs;
+    vt xv, b3v;
+    step_load<T, V>(flags, off, X, B3, xv, b3v);
     vt h = (vt)(T)0;
-    if (flags & kHasH) h = qsun_hrow<T, V>(q, (uint32_t)row, fixed_a, wg, sG);
-    step_epilogue<T, V>(flags, row, lay.rs, cofs, out_col0, cx, ch, cb, h, X, B3, OUT, out_ld, Pr,
-                        dacc);
+    if (flags & kHasH) h = qsun_hrow_dg<T, V>(q, (uint32_t)row, sG);
+    step_store<T, V>(flags, row, off, out_col0, cx, ch, cb, h, xv, b3v, OUT, out_ld, Pr, dacc);
   }
   if (flags & kDot) {
     const double t = block_sum_256(dacc, red);

@@ -5,10 +5,9 @@
 //   d(r) b2[r] + sum_{b != a} G~[a][b] b2[base | b] + sum_j c~_j b2[r ^ flip_j],
 // a = r mod 2^Lg, base = r - a, d(r) = (Dlo[r mod 2^lb] + Dhi[r >> lb] - shift)
 // * scale from the two diagonal tables built once per operator (FP64).  The
-// grain weights of a lane's fixed grain index live in registers, the scaled
-// outer couplings and flip masks in LDS; gathers use 32-bit byte offsets from
-// a uniform base (a shift and an XOR per flip when the row stride in bytes is
-// a power of two).
+// scaled grain matrix, outer couplings and flip masks sit in LDS; gathers use
+// 32-bit byte offsets from a uniform base (a shift and an XOR per flip when the
+// row stride in bytes is a power of two).
 #pragma once
 
 #include "pf_device.h"
@@ -17,6 +16,7 @@
 namespace pf {
 
 constexpr int kMaxDg = 1 << kMaxGrain;
+constexpr int kMaxLocPad = 32;  // outer spins padded to whole gather batches
 
 // Per-block constants of one Quantum-Sun operator in one step.
 struct QsunRow {
@@ -25,7 +25,7 @@ struct QsunRow {
   uint32_t rsb;          // row stride (bytes)
   int rsh;               // log2(rsb), or -1 when rsb is not a power of two
   int dg;                // 2^Lg
-  int nloc;              // outer spins
+  int nloc;              // outer spins, padded to a whole number of gather batches
   int lb;                // low bits of the diagonal split
   const double* dlo;     // Dlo[2^lb]
   const double* dhi;     // Dhi[2^(Lg+Lloc-lb)]
@@ -39,65 +39,92 @@ __device__ __forceinline__ uint32_t qsun_off(const QsunRow& q, uint32_t r) {
 }
 
 // Stage the per-block LDS tables: sF (flip masks, shifted), sC (T couplings),
-// sG (T grain matrix, dg x dg).  Call from all threads, then __syncthreads().
+// sG (T grain matrix, dg x dg).  The flip list is padded to npad entries with
+// zero masks and zero weights (a flip onto the row itself that adds nothing),
+// so the gather loop runs whole batches without guards.  Call from all
+// threads, then __syncthreads().
 template <typename T>
-__device__ __forceinline__ void qsun_stage(int nloc, int dg, const uint32_t* flip,
+__device__ __forceinline__ void qsun_stage(int nloc, int npad, int dg, const uint32_t* flip,
                                            const double* coup, const double* G, double scale,
                                            int rsh, uint32_t* sF, T* sC, T* sG) {
-  for (int j = threadIdx.x; j < nloc; j += blockDim.x) {
-    sF[j] = rsh >= 0 ? (flip[j] << rsh) : flip[j];
-    sC[j] = (T)(coup[j] * scale);
+  for (int j = threadIdx.x; j < npad; j += blockDim.x) {
+    const bool real = j < nloc;
+    sF[j] = real ? (rsh >= 0 ? (flip[j] << rsh) : flip[j]) : 0u;
+    sC[j] = real ? (T)(coup[j] * scale) : (T)0;
   }
   for (int t = threadIdx.x; t < dg * dg; t += blockDim.x) sG[t] = (T)(G[t] * scale);
 }
 
-// This lane's grain weights G~[a][b] (0 at b == a, whose term carries d(r)).
-template <typename T>
-__device__ __forceinline__ void qsun_lane_weights(const T* sG, int dg, int a, T (&wg)[kMaxDg]) {
-#pragma unroll
-  for (int b = 0; b < kMaxDg; ++b) wg[b] = (b < dg && b != a) ? sG[a * dg + b] : (T)0;
+// Outer spins rounded up to whole gather batches of nbf.
+__host__ __device__ __forceinline__ int qsun_pad(int nloc, int nbf) {
+  return (nloc + nbf - 1) / nbf * nbf;
 }
 
-// H~ b2 at row r for this lane's V columns.  fixed_a: the lane's grain index
-// is the same for every row it visits (rows per block pass divisible by dg),
-// so wg holds its weights; otherwise they are read from sG per row.
+// Gathers issued per batch before any is accumulated: about 128 bytes in
+// flight per lane and batch, between 2 and 16 gathers.
 template <typename T, int V>
+constexpr int qsun_flip_batch() {
+  return (128 / (V * (int)sizeof(T))) < 2 ? 2
+         : (128 / (V * (int)sizeof(T))) > 16 ? 16
+                                              : 128 / (V * (int)sizeof(T));
+}
+
+// H~ b2 at row r for this lane's V columns, DG = 2^Lg; sG holds the scaled
+// grain matrix (LDS).  Every load of a group (diagonal tables, grain block, a
+// batch of flips) is issued before the first of them is used.
+template <typename T, int V, int DG>
 __device__ __forceinline__ typename VT<T, V>::t qsun_hrow(const QsunRow& q, uint32_t r,
-                                                          bool fixed_a, const T (&wg)[kMaxDg],
                                                           const T* sG) {
   typedef typename VT<T, V>::t vt;
+  constexpr int NBF = qsun_flip_batch<T, V>();
   const T* sC = reinterpret_cast<const T*>(q.sC);
-  const int a = (int)(r & (uint32_t)(q.dg - 1));
-  const double dd = q.dlo[r & ((1u << q.lb) - 1u)] + q.dhi[r >> q.lb];
-  const T d = (T)((dd - q.shift) * q.scale);
-  const uint32_t roff = qsun_off(q, r);
-  vt acc = d * *reinterpret_cast<const vt*>(q.b2 + roff);
-  const uint32_t gbase = qsun_off(q, r & ~(uint32_t)(q.dg - 1));
+  const int a = (int)(r & (uint32_t)(DG - 1));
+  const double dl = q.dlo[r & ((1u << q.lb) - 1u)];
+  const double dh = q.dhi[r >> q.lb];
+  constexpr int GB = DG < NBF ? DG : NBF;  // grain loads per batch
+  const uint32_t gbase = qsun_off(q, r & ~(uint32_t)(DG - 1));
+  const vt xr = *reinterpret_cast<const vt*>(q.b2 + qsun_off(q, r));
+  vt acc = (vt)(T)0;
 #pragma unroll
-  for (int b = 0; b < kMaxDg; ++b) {
-    if (b < q.dg) {  // uniform
-      const T w = fixed_a ? wg[b] : (b == a ? (T)0 : sG[a * q.dg + b]);
-      acc += w * *reinterpret_cast<const vt*>(q.b2 + gbase + (uint32_t)b * q.rsb);
+  for (int b0 = 0; b0 < DG; b0 += GB) {
+    vt xg[GB];
+#pragma unroll
+    for (int b = 0; b < GB; ++b)
+      xg[b] = *reinterpret_cast<const vt*>(q.b2 + gbase + (uint32_t)(b0 + b) * q.rsb);
+#pragma unroll
+    for (int b = 0; b < GB; ++b) {
+      const int bb = b0 + b;
+      if (bb != a) acc += sG[a * DG + bb] * xg[b];
     }
   }
   const uint32_t rs_r = q.rsh >= 0 ? (r << q.rsh) : r;
-  for (int j0 = 0; j0 < q.nloc; j0 += kGatherBatch) {
-    vt x[kGatherBatch];
+  for (int j0 = 0; j0 < q.nloc; j0 += NBF) {  // q.nloc is a multiple of NBF
+    vt x[NBF];
 #pragma unroll
-    for (int u = 0; u < kGatherBatch; ++u) {
+    for (int u = 0; u < NBF; ++u) {
       const int j = j0 + u;
-      x[u] = (vt)(T)0;
-      if (j < q.nloc) {  // uniform
-        const uint32_t o = q.rsh >= 0 ? ((rs_r ^ q.sF[j]) + q.cofs_b)
-                                      : ((r ^ q.sF[j]) * q.rsb + q.cofs_b);
-        x[u] = *reinterpret_cast<const vt*>(q.b2 + o);
-      }
+      const uint32_t o = q.rsh >= 0 ? ((rs_r ^ q.sF[j]) + q.cofs_b)
+                                    : ((r ^ q.sF[j]) * q.rsb + q.cofs_b);
+      x[u] = *reinterpret_cast<const vt*>(q.b2 + o);
     }
 #pragma unroll
-    for (int u = 0; u < kGatherBatch; ++u)
-      if (j0 + u < q.nloc) acc += sC[j0 + u] * x[u];
+    for (int u = 0; u < NBF; ++u) acc += sC[j0 + u] * x[u];
   }
+  const T d = (T)((dl + dh - q.shift) * q.scale);
+  acc += d * xr;
   return acc;
+}
+
+// qsun_hrow for the runtime grain size q.dg (2, 4, 8 or 16).
+template <typename T, int V>
+__device__ __forceinline__ typename VT<T, V>::t qsun_hrow_dg(const QsunRow& q, uint32_t r,
+                                                             const T* sG) {
+  switch (q.dg) {  // uniform
+    case 2: return qsun_hrow<T, V, 2>(q, r, sG);
+    case 4: return qsun_hrow<T, V, 4>(q, r, sG);
+    case 16: return qsun_hrow<T, V, 16>(q, r, sG);
+    default: return qsun_hrow<T, V, 8>(q, r, sG);
+  }
 }
 
 }  // namespace pf

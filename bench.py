@@ -197,9 +197,9 @@ def qsun_batch_instances(dev, n=64):
 def qsun_batch(args):
     """Batched workload: nev=32 pairs of each of 64 Quantum-Sun instances at
     the middle of the spectrum (single-vector Lanczos, eigentol 1e-10).
-    Timed: the batched mixed-precision solve (FP32 filters once a pair set
-    is nearly converged), the batched FP64 solve, and with --plain the plain
-    FP64 path (one polfed call per instance)."""
+    Timed, as selected by --modes: the batched mixed-precision solve (FP32
+    filters once an instance's wanted pairs are nearly converged), the batched
+    FP64 solve, and the plain FP64 path (one polfed call per instance)."""
     import numpy as np
     import torch
     from polfed_amd import FactorizationConfig, TransformConfig, polfed, polfed_batch
@@ -220,8 +220,9 @@ def qsun_batch(args):
                 "iterations_max": int(max(r.iterations for _, _, r in res_list)),
                 "orders": [int(r.order) for _, _, r in res_list]}
 
+    modes = [m for m in args.modes.split(",") if m]
     results = {}
-    for mode in ("mixed", "f64"):
+    for mode in [m for m in ("mixed", "f64") if m in modes]:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res, brep = polfed_batch(ops, x0s, 32, "middle", produce_report=True, fact=fact,
@@ -237,16 +238,16 @@ def qsun_batch(args):
         e.pop("orders")
         out[f"batched_{mode}"] = e
         log(f"batched {mode}: {t:.1f} s pairs={e['pairs_total']} maxres={e['max_residual']:.2e}")
-    ref = results["f64"]
-    errs = []
-    for (v1, _, r1), (v2, _, r2) in zip(results["mixed"], ref):
-        n = min(len(v1), len(v2))
-        if n:
-            errs.append(float(np.max(np.abs(v1[:n] - v2[:n])) / (r2.Emax - r2.Emin)))
-    out["batched_mixed"]["max_eig_err_vs_fp64_over_width"] = max(errs) if errs else None
-    out["batched_mixed"]["speedup_vs_batched_fp64"] = (out["batched_f64"]["time_s"]
-                                                       / out["batched_mixed"]["time_s"])
-    if args.plain:
+    if "mixed" in results and "f64" in results:
+        errs = []
+        for (v1, _, r1), (v2, _, r2) in zip(results["mixed"], results["f64"]):
+            n = min(len(v1), len(v2))
+            if n:
+                errs.append(float(np.max(np.abs(v1[:n] - v2[:n])) / (r2.Emax - r2.Emin)))
+        out["batched_mixed"]["max_eig_err_vs_fp64_over_width"] = max(errs) if errs else None
+        out["batched_mixed"]["speedup_vs_batched_fp64"] = (out["batched_f64"]["time_s"]
+                                                           / out["batched_mixed"]["time_s"])
+    if "plain" in modes:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         plain = []
@@ -259,7 +260,9 @@ def qsun_batch(args):
         e = summarize(plain, t)
         e.pop("orders")
         out["plain_f64"] = e
-        out["batched_mixed"]["speedup_vs_plain_fp64"] = t / out["batched_mixed"]["time_s"]
+        for m in ("mixed", "f64"):
+            if f"batched_{m}" in out:
+                out[f"batched_{m}"]["speedup_vs_plain_fp64"] = t / out[f"batched_{m}"]["time_s"]
         log(f"plain f64: {t:.1f} s pairs={e['pairs_total']}")
     print(json.dumps(out), flush=True)
 
@@ -451,8 +454,9 @@ def main():
                     help="xxz-l20: the bench line; qsun-batch: 64 Quantum-Sun instances; "
                          "xxz-l22: FP32/FP64 precision report; j1j2-l24: KPM and filter at L=24")
     ap.add_argument("--instances", type=int, default=64, help="qsun-batch: number of instances")
-    ap.add_argument("--plain", action="store_true",
-                    help="qsun-batch: also time the plain FP64 path, one instance at a time")
+    ap.add_argument("--modes", default="mixed,f64",
+                    help="qsun-batch: comma list of mixed, f64 (batched) and plain (FP64, "
+                         "one polfed call per instance)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="profiling driver: one 256 MiB device copy, five times")
     args = ap.parse_args()

@@ -94,6 +94,23 @@ def build_workload(dev, fp="f64"):
     return op, x0
 
 
+def progress_filter(op, label, every=5):
+    """Log one line per `every` filter applications (long solves stay audible)."""
+    inner = op.filter
+    state = {"n": 0, "t0": time.perf_counter()}
+
+    def wrapped(coeffs, X, Y=None, dtype="f64", polytype="Chebyshev"):
+        out = inner(coeffs, X, Y, dtype=dtype, polytype=polytype)
+        state["n"] += 1
+        if state["n"] % every == 0:
+            log(f"{label}: filter application {state['n']} ({dtype}), "
+                f"{time.perf_counter() - state['t0']:.1f} s")
+        return out
+
+    op.filter = wrapped
+    return inner
+
+
 def kernel_name(op, dtype="f64", s=8):
     """Template instance of the step kernel the autotuner picked for (dtype, s)."""
     tuned = getattr(op, "_tuned", {}).get((dtype, s))
@@ -284,6 +301,7 @@ def xxz_l22(args):
     out = {"metric": METRIC, "workload": "XXZ L=22 Sz=0 D=705432 W=2 seed 2, block 16, "
            "nev=200, target middle, eigentol 1e-10", "unit": "s", "higher_is_better": False}
     ref = None
+    progress_filter(op, "l22")
     for mode in ("f64", "mixed", "f32"):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -333,9 +351,21 @@ def j1j2_l24(args):
     from polfed_amd.core import _extremal
     v0 = dev.to_device(np.random.default_rng(5).standard_normal(op.D))
     v0 = v0 / torch.linalg.vector_norm(v0)
+    inner_apply = op.apply
+    napply = {"n": 0}
+
+    def apply_logged(X, Y=None, rescaled=False, dtype="f64"):
+        napply["n"] += 1
+        if napply["n"] % 100 == 0:
+            log(f"l24 bounds: {napply['n']} operator applications")
+        return inner_apply(X, Y, rescaled=rescaled, dtype=dtype)
+
+    op.apply = apply_logged
     t0 = time.perf_counter()
     Emin, Emax = _extremal(dev, op, v0, "SR"), _extremal(dev, op, v0, "LR")
     t_bounds = time.perf_counter() - t0
+    op.apply = inner_apply
+    log(f"l24 bounds [{Emin:.6f}, {Emax:.6f}] in {t_bounds:.1f} s")
     op.set_rescale(Emin, Emax)
     P = bo.fill_probes(dev, op.D, 32, seed=24, kind="phase")
     out = {"metric": METRIC, "workload": "J1-J2 L=24 Sz=0 D=2704156 J1=1 J2=0.5; KPM R=32 "

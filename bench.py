@@ -121,15 +121,19 @@ def kernel_name(op, dtype="f64", s=8):
 
 
 def measured_traffic(kernel):
-    """HBM-side bytes per launch of `kernel` on this workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter scaled by
-    its plain-copy calibration factor); None when that kernel was not counted."""
+    """(full kernel name, HBM-side bytes per launch) of the step kernel the
+    autotuner picked, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (each counter scaled by its plain-copy calibration factor), when
+    the counted instance has the same element type and lanes layout as
+    `kernel`; (kernel, None) otherwise."""
     try:
         table = json.loads(TRAFFIC_FILE.read_text())
-    except (OSError, ValueError):
-        return None
-    entry = table.get("kernels", {}).get(kernel or "")
-    return None if entry is None else float(entry["bytes_per_launch"])
+        sel = table["selected"]
+    except (OSError, ValueError, KeyError):
+        return kernel, None
+    if kernel and sel["kernel"].startswith(kernel[:-1] + ","):
+        return sel["kernel"], float(sel["bytes_per_launch"])
+    return kernel, None
 
 
 def filter_only(args):
@@ -534,7 +538,7 @@ def main():
     ms_step, bytes_step, nsteps, filter_ms = timer.summary(op.D, 8)
     rep = reports[-1]
     achieved = bytes_step / (ms_step * 1e-3) / 1e9
-    kname = kernel_name(op)
+    kname, traffic = measured_traffic(kernel_name(op))
     out = {
         "metric": METRIC,
         "value": elapsed / args.steps,
@@ -555,7 +559,7 @@ def main():
                   "Emin": rep.Emin, "Emax": rep.Emax, "times_s": rep.times},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(kname), "kernel": kname,
+                     "traffic": traffic, "kernel": kname,
                      "per_launch_bytes": bytes_step, "avg_launch_ms": ms_step,
                      "filter_share": filter_ms * 1e-3 / elapsed},
     }

@@ -156,7 +156,8 @@ enum {
   PF_TUNE_RESET = 3,
   PF_TUNE_NONTEMPORAL = 4,
   PF_TUNE_PBLOCKS = 5,
-  PF_TUNE_WIDE = 6
+  PF_TUNE_WIDE = 6,
+  PF_TUNE_GRAIN_SHUFFLE = 7
 };
 
 /* Force a launch knob of the row-interleaved step kernels for this operator:
@@ -166,7 +167,9 @@ enum {
  * vectors), PF_TUNE_PBLOCKS (-1 default, 0: one block per row group, n: a
  * persistent grid of n blocks per CU, each over a contiguous range of rows),
  * PF_TUNE_WIDE (-1 default, 0/1: twice the neighbour gathers in flight per
- * batch in the spin-chain kernel with rank tables),
+ * batch in the spin-chain kernel with rank tables), PF_TUNE_GRAIN_SHUFFLE
+ * (-1 default, 0/1: Quantum-Sun grain block by cross-lane shuffles or by
+ * reloads),
  * PF_TUNE_RESET (drop forced and autotuned settings).  No reference
  * counterpart: launch tuning of this library. */
 int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value);
@@ -279,14 +282,16 @@ enum {
   PF_BATCH_GRAPH_STEPS = 2,    /* step nodes per captured graph (default 96)      */
   PF_BATCH_VEC = 3,            /* columns per lane: 1/2/4/8 dividing ncols        */
   PF_BATCH_ROWS_PER_LANE = 4,  /* rows per lane: 1..64                            */
-  PF_BATCH_XCD = 5             /* 1: XCD-contiguous block mapping (default)       */
+  PF_BATCH_XCD = 5,            /* 1: XCD-contiguous block mapping (default)       */
+  PF_BATCH_GRAIN_SHUFFLE = 6   /* 1: grain block by cross-lane shuffles (default) */
 };
 /* Launch settings of the batch.  No reference counterpart. */
 int pf_batch_set_tuning(pf_batch* b, int32_t key, int32_t value);
 
-/* Times every (columns per lane, rows per lane, XCD mapping) tiling of the
- * multi-instance step kernel on all instances in arithmetic `dtype`, then
- * graph replays against stream launches, and keeps the fastest.  best_us
+/* Times every (columns per lane, rows per lane, XCD mapping, grain-block
+ * shuffles or reloads) tiling of the multi-instance step kernel on all
+ * instances in arithmetic `dtype`, then graph replays against stream
+ * launches, and keeps the fastest.  best_us
  * (host, may be NULL): time per step of the whole batch.  Overwrites the
  * batch's work buffers.  No reference counterpart. */
 int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, double* best_us,

@@ -32,12 +32,14 @@ PF_TUNE_RESET = 3
 PF_TUNE_NONTEMPORAL = 4
 PF_TUNE_PBLOCKS = 5
 PF_TUNE_WIDE = 6
+PF_TUNE_GRAIN_SHUFFLE = 7
 
 PF_BATCH_GRAPH = 1
 PF_BATCH_GRAPH_STEPS = 2
 PF_BATCH_VEC = 3
 PF_BATCH_ROWS_PER_LANE = 4
 PF_BATCH_XCD = 5
+PF_BATCH_GRAIN_SHUFFLE = 6
 
 PF_PROBE_GAUSSIAN = 0
 PF_PROBE_PHASE = 1

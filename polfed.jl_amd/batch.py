@@ -23,7 +23,7 @@ from .operators import QSunOperator, _dtype
 
 _TUNE = {"graph": _lib.PF_BATCH_GRAPH, "graph_steps": _lib.PF_BATCH_GRAPH_STEPS,
          "vec": _lib.PF_BATCH_VEC, "rows_per_lane": _lib.PF_BATCH_ROWS_PER_LANE,
-         "xcd": _lib.PF_BATCH_XCD}
+         "xcd": _lib.PF_BATCH_XCD, "grain_shuffle": _lib.PF_BATCH_GRAIN_SHUFFLE}
 
 
 class QSunBatch:
@@ -108,7 +108,7 @@ class QSunBatch:
                                        i64(ldy), self.device.stream))
 
     def set_tuning(self, **kw) -> None:
-        """graph (0/1), graph_steps, vec, rows_per_lane, xcd."""
+        """graph (0/1), graph_steps, vec, rows_per_lane, xcd, grain_shuffle (0/1)."""
         for k, v in kw.items():
             if k not in _TUNE:
                 raise ValueError(f"unknown batch tuning key {k!r}")

@@ -70,6 +70,7 @@ struct BatchArgs {
   int rpb;   // rows per block pass (256 / G)
   int rpt;   // block passes (rows per lane)
   int xcd;   // XCD-contiguous block mapping
+  int shfl;  // 1: grain block by cross-lane shuffles, 0: by reloads
 };
 
 __device__ __forceinline__ void poly_ab_dev(int type, int k, double& alpha, double& beta_k) {
@@ -144,6 +145,7 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& 
   q.shift = I.shift;
   q.sF = sF;
   q.sC = sC;
+  const int G_ = A.shfl ? A.G : 0;  // 0 selects the reload path
   for (int u = 0; u < A.rpt; ++u) {
     const int64_t row = (lb * A.rpt + u) * A.rpb + rloc;
     if (row >= I.D) break;
@@ -151,7 +153,7 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& 
     const vt xv = vload<T, V>(X + o);
     const vt b3v = hasB3 ? vload<T, V>(B3 + o) : (vt)(T)0;
     vt hv = (vt)(T)0;
-    if (hasH) hv = qsun_hrow_dg<T, V>(q, (uint32_t)row, sG);
+    if (hasH) hv = qsun_hrow_dg<T, V>(q, (uint32_t)row, sG, G_);
     vt out = cx * xv;
     if (hasH) out += ch * hv;
     if (hasB3) out += cb * b3v;
@@ -221,7 +223,7 @@ struct pf_batch {
   bool have_filter = false;
   DevBuf insts, dyn, blk0d, coefs, kstate, bufs;
   // tuning
-  int V = 1, rpt = 1, xcd = 1, use_graph = 1, gsteps = 96;
+  int V = 1, rpt = 1, xcd = 1, use_graph = 1, gsteps = 96, shfl = 1;
   // pinned staging of the per-application records; the event marks the end
   // of the last copy out of it
   BatchDyn* hdyn = nullptr;
@@ -321,6 +323,7 @@ struct pf_batch {
     A.rpb = rpb();
     A.rpt = rpt;
     A.xcd = xcd;
+    A.shfl = shfl;
     return A;
   }
   template <typename T>
@@ -552,6 +555,11 @@ extern "C" int pf_batch_set_tuning(pf_batch* b, int32_t key, int32_t value) {
         b->xcd = value;
         b->drop_graph();
         break;
+      case PF_BATCH_GRAIN_SHUFFLE:
+        PF_REQUIRE(value == 0 || value == 1, "batch: grain shuffle 0/1");
+        b->shfl = value;
+        b->drop_graph();
+        break;
       default: throw Error{PF_ERR_INVALID, "batch: unknown tuning key"};
     }
   });
@@ -580,21 +588,23 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
     PF_HIP(hipEventCreate(&e0));
     PF_HIP(hipEventCreate(&e1));
     struct Cand {
-      int V, rpt, xcd;
+      int V, rpt, xcd, shfl;
       double us;
     };
     std::vector<Cand> cands;
     for (int V : {1, 2, 4, 8})
       if (batch_vec_ok(b->s, V) && !(dtype == PF_F64 && V > 8))
         for (int rpt : {1, 4, 16, 64})
-          for (int x : {1, 0}) cands.push_back({V, rpt, x, 1e300});
-    const int saveV = b->V, saveR = b->rpt, saveX = b->xcd;
+          for (int x : {1, 0})
+            for (int sh : {1, 0}) cands.push_back({V, rpt, x, sh, 1e300});
+    const int saveV = b->V, saveR = b->rpt, saveX = b->xcd, saveS = b->shfl;
     const bool had_filter = b->have_filter;
     for (int round = 0; round < 3; ++round) {
       for (auto& c : cands) {
         b->V = c.V;
         b->rpt = c.rpt;
         b->xcd = c.xcd;
+        b->shfl = c.shfl;
         b->layout_blocks();
         BatchArgs A = b->args();
         A.coefs = nullptr;
@@ -611,7 +621,7 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
       }
     }
     double best = 1e300;
-    Cand bc{saveV, saveR, saveX, 1e300};
+    Cand bc{saveV, saveR, saveX, saveS, 1e300};
     for (const auto& c : cands)
       if (c.us < 0.99 * best) {
         best = c.us;
@@ -620,6 +630,7 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
     b->V = bc.V;
     b->rpt = bc.rpt;
     b->xcd = bc.xcd;
+    b->shfl = bc.shfl;
     b->layout_blocks();
     // graph replays against stream launches for the chosen tiling
     {

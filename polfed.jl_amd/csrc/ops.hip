@@ -309,6 +309,7 @@ __global__ __launch_bounds__(256) void qsun_step_kernel(
   q.shift = af.shift;
   q.sF = sF;
   q.sC = sC;
+  const int G_ = lay.shfl ? lay.G : 0;  // 0 selects the reload path
   double dacc = 0.0;
 
   const int64_t nblk = gridDim.x;
@@ -321,7 +322,7 @@ __global__ __launch_bounds__(256) void qsun_step_kernel(
     vt xv, b3v;
     step_load<T, V>(flags, off, X, B3, xv, b3v);
     vt h = (vt)(T)0;
-    if (flags & kHasH) h = qsun_hrow_dg<T, V>(q, (uint32_t)row, sG);
+    if (flags & kHasH) h = qsun_hrow_dg<T, V>(q, (uint32_t)row, sG, G_);
     step_store<T, V>(flags, row, off, out_col0, cx, ch, cb, h, xv, b3v, OUT, out_ld, Pr, dacc);
   }
   if (flags & kDot) {
@@ -339,7 +340,7 @@ static bool valid_vec(int dtype, int s, int V) {
   return s / V <= 256;
 }
 
-Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt, int pb, int wide) {
+Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt, int pb, int wide, int shfl) {
   Layout l{};
   if (V == 0 || !valid_vec(dtype, s, V)) {
     if (dtype == PF_F64)
@@ -356,6 +357,7 @@ Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt, int pb, int 
   l.nt = nt;
   l.pblocks = pb;
   l.wide = wide;
+  l.shfl = shfl;
   return l;
 }
 
@@ -375,8 +377,10 @@ Layout tuned_layout(pf_op* op, int dtype, int s) {
   if (op->force.nt >= 0) c.nt = op->force.nt;
   if (op->force.pb >= 0) c.pb = op->force.pb;
   if (op->force.wide >= 0) c.wide = op->force.wide;
+  if (op->force.shfl >= 0) c.shfl = op->force.shfl;
   return interleaved_layout(dtype, s, c.V, c.xcd < 0 ? 1 : c.xcd, c.nt < 0 ? 1 : c.nt,
-                            c.pb < 0 ? default_pblocks(op) : c.pb, c.wide < 0 ? 0 : c.wide);
+                            c.pb < 0 ? default_pblocks(op) : c.pb, c.wide < 0 ? 0 : c.wide,
+                            c.shfl < 0 ? 1 : c.shfl);
 }
 
 Layout colmajor_layout(int64_t ld) {
@@ -1179,6 +1183,9 @@ extern "C" int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value) {
     } else if (key == PF_TUNE_NONTEMPORAL) {
       PF_REQUIRE(value >= -1 && value <= 1, "tuning: non-temporal is -1, 0 or 1");
       op->force.nt = value;
+    } else if (key == PF_TUNE_GRAIN_SHUFFLE) {
+      PF_REQUIRE(value >= -1 && value <= 1, "tuning: grain shuffle is -1, 0 or 1");
+      op->force.shfl = value;
     } else if (key == PF_TUNE_WIDE) {
       PF_REQUIRE(value >= -1 && value <= 1, "tuning: wide gather batches is -1, 0 or 1");
       op->force.wide = value;
@@ -1226,7 +1233,9 @@ extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nste
     const int pbs[3] = {default_pblocks(op), 4, 0};
     for (int vi = 0; vi < 6; ++vi) {
       if (!valid_vec(dtype, ncols, vs[vi])) continue;
-      const int nwide = (op->kind == kChain && op->chain.tsplit > 0) ? 2 : 1;
+      // second variant: wide gather batches (chains with tables) or grain
+      // reloads instead of shuffles (Quantum Sun)
+      const int nwide = ((op->kind == kChain && op->chain.tsplit > 0) || op->kind == kQsun) ? 2 : 1;
       for (int pi = 0; pi < 3; ++pi) {
         if (pi > 0 && pbs[pi] == pbs[0]) continue;
         for (int x = 1; x >= 0; --x) {
@@ -1236,7 +1245,8 @@ extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nste
             c.xcd = x;
             c.nt = 1;
             c.pb = pbs[pi];
-            c.wide = w;
+            c.wide = op->kind == kChain ? w : 0;
+            c.shfl = op->kind == kQsun ? 1 - w : 1;
             cands.push_back({c, 1e300});
           }
         }
@@ -1244,8 +1254,8 @@ extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nste
     }
     for (int round = 0; round < 3; ++round) {
       for (auto& cd : cands) {
-        const Layout lay =
-            interleaved_layout(dtype, ncols, cd.c.V, cd.c.xcd, cd.c.nt, cd.c.pb, cd.c.wide);
+        const Layout lay = interleaved_layout(dtype, ncols, cd.c.V, cd.c.xcd, cd.c.nt, cd.c.pb,
+                                              cd.c.wide, cd.c.shfl);
         for (int w = 0; w < 2; ++w)
           launch_step(op, dtype, kHasX | kHasH | kHasB3, lay, ncols, af, 0.5, 2.0, -1.0, b[0],
                       b[1], b[2], b[3], 0, st);

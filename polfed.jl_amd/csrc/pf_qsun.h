@@ -55,9 +55,23 @@ __device__ __forceinline__ void qsun_stage(int nloc, int npad, int dg, const uin
   for (int t = threadIdx.x; t < dg * dg; t += blockDim.x) sG[t] = (T)(G[t] * scale);
 }
 
-// Outer spins rounded up to whole gather batches of nbf.
+// Outer spins rounded up to whole gather batches of nbf (the padding flips
+// are zero-weight reloads of the row itself, cheaper than guarded batches).
 __host__ __device__ __forceinline__ int qsun_pad(int nloc, int nbf) {
   return (nloc + nbf - 1) / nbf * nbf;
+}
+
+// x from lane (this lane ^ mask), component-wise (all lanes of the group active).
+template <typename T, int V>
+__device__ __forceinline__ typename VT<T, V>::t shfl_xor_vec(typename VT<T, V>::t x, int mask) {
+  if constexpr (V == 1) {
+    return __shfl_xor(x, mask, 64);
+  } else {
+    typename VT<T, V>::t y;
+#pragma unroll
+    for (int v = 0; v < V; ++v) y[v] = __shfl_xor(x[v], mask, 64);
+    return y;
+  }
 }
 
 // Gathers issued per batch before any is accumulated: about 128 bytes in
@@ -74,31 +88,19 @@ constexpr int qsun_flip_batch() {
 // batch of flips) is issued before the first of them is used.
 template <typename T, int V, int DG>
 __device__ __forceinline__ typename VT<T, V>::t qsun_hrow(const QsunRow& q, uint32_t r,
-                                                          const T* sG) {
+                                                          const T* sG, int G) {
   typedef typename VT<T, V>::t vt;
   constexpr int NBF = qsun_flip_batch<T, V>();
+  constexpr int GB = DG < NBF ? DG : NBF;  // grain loads per batch (load path)
   const T* sC = reinterpret_cast<const T*>(q.sC);
   const int a = (int)(r & (uint32_t)(DG - 1));
   const double dl = q.dlo[r & ((1u << q.lb) - 1u)];
   const double dh = q.dhi[r >> q.lb];
-  constexpr int GB = DG < NBF ? DG : NBF;  // grain loads per batch
-  const uint32_t gbase = qsun_off(q, r & ~(uint32_t)(DG - 1));
   const vt xr = *reinterpret_cast<const vt*>(q.b2 + qsun_off(q, r));
-  vt acc = (vt)(T)0;
-#pragma unroll
-  for (int b0 = 0; b0 < DG; b0 += GB) {
-    vt xg[GB];
-#pragma unroll
-    for (int b = 0; b < GB; ++b)
-      xg[b] = *reinterpret_cast<const vt*>(q.b2 + gbase + (uint32_t)(b0 + b) * q.rsb);
-#pragma unroll
-    for (int b = 0; b < GB; ++b) {
-      const int bb = b0 + b;
-      if (bb != a) acc += sG[a * DG + bb] * xg[b];
-    }
-  }
   const uint32_t rs_r = q.rsh >= 0 ? (r << q.rsh) : r;
-  for (int j0 = 0; j0 < q.nloc; j0 += NBF) {  // q.nloc is a multiple of NBF
+  vt acc = (vt)(T)0;
+  // outer flips: whole batches of NBF (q.nloc is padded to a multiple of NBF)
+  for (int j0 = 0; j0 < q.nloc; j0 += NBF) {
     vt x[NBF];
 #pragma unroll
     for (int u = 0; u < NBF; ++u) {
@@ -110,20 +112,43 @@ __device__ __forceinline__ typename VT<T, V>::t qsun_hrow(const QsunRow& q, uint
 #pragma unroll
     for (int u = 0; u < NBF; ++u) acc += sC[j0 + u] * x[u];
   }
+  // grain block: with G a power of two its 2^Lg rows sit in lanes whose
+  // indices differ by (a ^ a') * G (all active: a pass holds whole blocks), so
+  // the rows' own values are shuffled instead of reloaded
+  if (G > 0 && (G & (G - 1)) == 0 && DG * G <= 64) {
+#pragma unroll
+    for (int b = 1; b < DG; ++b) acc += sG[a * DG + (a ^ b)] * shfl_xor_vec<T, V>(xr, b * G);
+  } else {
+    const uint32_t gbase = qsun_off(q, r & ~(uint32_t)(DG - 1));
+#pragma unroll
+    for (int b0 = 0; b0 < DG; b0 += GB) {
+      vt xg[GB];
+#pragma unroll
+      for (int b = 0; b < GB; ++b)
+        xg[b] = *reinterpret_cast<const vt*>(q.b2 + gbase + (uint32_t)(b0 + b) * q.rsb);
+#pragma unroll
+      for (int b = 0; b < GB; ++b) {
+        const int bb = b0 + b;
+        if (bb != a) acc += sG[a * DG + bb] * xg[b];
+      }
+    }
+  }
   const T d = (T)((dl + dh - q.shift) * q.scale);
   acc += d * xr;
   return acc;
 }
 
-// qsun_hrow for the runtime grain size q.dg (2, 4, 8 or 16).
+// qsun_hrow for the runtime grain size q.dg (2, 4, 8 or 16), G lanes per row.
+// Every lane of a grain block must take part (rows per pass are whole
+// blocks and D is a multiple of 2^Lg).
 template <typename T, int V>
 __device__ __forceinline__ typename VT<T, V>::t qsun_hrow_dg(const QsunRow& q, uint32_t r,
-                                                             const T* sG) {
+                                                             const T* sG, int G) {
   switch (q.dg) {  // uniform
-    case 2: return qsun_hrow<T, V, 2>(q, r, sG);
-    case 4: return qsun_hrow<T, V, 4>(q, r, sG);
-    case 16: return qsun_hrow<T, V, 16>(q, r, sG);
-    default: return qsun_hrow<T, V, 8>(q, r, sG);
+    case 2: return qsun_hrow<T, V, 2>(q, r, sG, G);
+    case 4: return qsun_hrow<T, V, 4>(q, r, sG, G);
+    case 16: return qsun_hrow<T, V, 16>(q, r, sG, G);
+    default: return qsun_hrow<T, V, 8>(q, r, sG, G);
   }
 }
 

@@ -131,7 +131,8 @@ class Operator:
         return mu
 
     def set_tuning(self, xcd: int | None = None, vec: int | None = None, nt: int | None = None,
-                   pblocks: int | None = None, wide: int | None = None, reset: bool = False):
+                   pblocks: int | None = None, wide: int | None = None,
+                   grain_shuffle: int | None = None, reset: bool = False):
         """Force launch knobs of the row-interleaved kernels (see pf_op_set_tuning)."""
         if reset:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_RESET, 0))
@@ -140,6 +141,8 @@ class Operator:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_PBLOCKS, int(pblocks)))
         if wide is not None:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_WIDE, int(wide)))
+        if grain_shuffle is not None:
+            check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_GRAIN_SHUFFLE, int(grain_shuffle)))
         if xcd is not None:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_XCD_REMAP, int(xcd)))
         if vec is not None:

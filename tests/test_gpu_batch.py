@@ -38,9 +38,10 @@ def _oracle_filter(H, op, c, X, polytype="Chebyshev"):
     return oc.clenshaw_apply(lambda Z: Ht @ Z, c, X, polytype)
 
 
-@pytest.mark.parametrize("s,rows_per_lane", [(1, 1), (3, 1), (1, 16), (2, 64)])
+@pytest.mark.parametrize("s,rows_per_lane,shuffle", [(1, 1, 1), (3, 1, 1), (1, 16, 0), (2, 64, 1),
+                                                      (2, 4, 0)])
 @pytest.mark.parametrize("graph", [1, 0])
-def test_batch_filter_matches_oracle(dev, s, rows_per_lane, graph):
+def test_batch_filter_matches_oracle(dev, s, rows_per_lane, shuffle, graph):
     from polfed_amd.batch import QSunBatch
     ops, Hs = _instances(dev)
     orders = [37, 120, 0, 1, 64]      # uneven orders, including 0 and 1
@@ -50,7 +51,7 @@ def test_batch_filter_matches_oracle(dev, s, rows_per_lane, graph):
     batch = QSunBatch(ops, s)
     batch.set_filter(coeffs)
     batch.set_tuning(graph=graph, graph_steps=16,   # several replays per application
-                     rows_per_lane=rows_per_lane)
+                     rows_per_lane=rows_per_lane, grain_shuffle=shuffle)
     rng = np.random.default_rng(3)
     Xs = [rng.standard_normal((op.D, s)) for op in ops]
     refs = [_oracle_filter(H, op, c, X) for H, op, c, X in zip(Hs, ops, coeffs, Xs)]

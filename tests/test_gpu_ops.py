@@ -124,6 +124,27 @@ def test_filter_persistent_grid_settings_agree(dev, pblocks, wide, vec):
     assert _rel(Y, oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)) < 1e-12
 
 
+@pytest.mark.parametrize("L_grain,L_loc", [(1, 7), (2, 6), (4, 5)])
+@pytest.mark.parametrize("s", [1, 3, 8, 16])
+@pytest.mark.parametrize("shuffle", [1, 0])
+def test_qsun_grain_sizes_filter(dev, L_grain, L_loc, s, shuffle):
+    """Grain blocks of 2, 4 and 16 rows with 1, 3, 8 and 16 columns: the
+    cross-lane shuffle path (power-of-two lanes per row) and the load path
+    (3 lanes per row) against the oracle, FP64 and FP32."""
+    op, H = _qsun(dev, L_loc, L_grain, 20 + L_grain)
+    op.set_tuning(grain_shuffle=shuffle)
+    w = np.linalg.eigvalsh(H.toarray())
+    op.set_rescale(w[0] - 0.01, w[-1] + 0.01)
+    a, b = (op.Emax - op.Emin) / 2, (op.Emax + op.Emin) / 2
+    Ht = (H - b * sp.identity(H.shape[0])) / a
+    c = oc.filter_coefficients(0.2, 40)
+    X = np.random.default_rng(s).standard_normal((op.D, s))
+    ref = oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)
+    assert _rel(dev.to_host(op.filter(c, dev.to_device(X))), ref) < 1e-12
+    assert _rel(dev.to_host(op.filter(c, dev.to_device(X), dtype="f32")), ref) < 5e-4
+    assert _rel(dev.to_host(op.apply(dev.to_device(X))), H @ X) < 1e-13
+
+
 @pytest.mark.parametrize("s", [1, 3, 8])
 @pytest.mark.parametrize("kind", ["chain", "qsun"])
 def test_apply_matches_sparse(dev, kind, s):

@@ -157,7 +157,8 @@ enum {
   PF_TUNE_NONTEMPORAL = 4,
   PF_TUNE_PBLOCKS = 5,
   PF_TUNE_WIDE = 6,
-  PF_TUNE_GRAIN_SHUFFLE = 7
+  PF_TUNE_GRAIN_SHUFFLE = 7,
+  PF_TUNE_OCCUPANCY = 8
 };
 
 /* Force a launch knob of the row-interleaved step kernels for this operator:
@@ -169,7 +170,8 @@ enum {
  * PF_TUNE_WIDE (-1 default, 0/1: twice the neighbour gathers in flight per
  * batch in the spin-chain kernel with rank tables), PF_TUNE_GRAIN_SHUFFLE
  * (-1 default, 0/1: Quantum-Sun grain block by cross-lane shuffles or by
- * reloads),
+ * reloads), PF_TUNE_OCCUPANCY (-1 default, 0/1: spin-chain kernel built for
+ * 8 waves per SIMD),
  * PF_TUNE_RESET (drop forced and autotuned settings).  No reference
  * counterpart: launch tuning of this library. */
 int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value);

@@ -158,12 +158,12 @@ __device__ __forceinline__ void bond_ranks(const RankCtx& R, int nb, int b0, uin
 // contiguous row groups [p*n/nblk, (p+1)*n/nblk): with a persistent grid the
 // rank tables are staged in LDS once per block, not once per row group.
 template <typename T, int V, int GM, bool TBL, int NBW>
-__global__ __launch_bounds__(256) void chain_step_kernel(
-    const ChainParams P, const uint32_t* __restrict__ states, const uint32_t* __restrict__ binom,
+__device__ __forceinline__ void chain_step_body(
+    const ChainParams& P, const uint32_t* __restrict__ states, const uint32_t* __restrict__ binom,
     const uint4* __restrict__ rtab, const int rtab_vec, const double* __restrict__ diag,
     const Affine af, const int flags, const T cx, const T ch, const T cb,
     const T* __restrict__ X, const T* __restrict__ B2, const T* __restrict__ B3,
-    void* __restrict__ OUT, const Layout lay, const int64_t out_ld, double* __restrict__ dot_part,
+    void* __restrict__ OUT, const Layout& lay, const int64_t out_ld, double* __restrict__ dot_part,
     const T* __restrict__ Pr, const int64_t ngroups) {
   typedef typename VT<T, V>::t vt;
   constexpr int GE = GM < 1 ? 1 : GM;
@@ -269,6 +269,33 @@ __global__ __launch_bounds__(256) void chain_step_kernel(
   }
 }
 
+#define PF_CHAIN_ARGS                                                                             \
+  const ChainParams P, const uint32_t* __restrict__ states, const uint32_t* __restrict__ binom,  \
+      const uint4* __restrict__ rtab, const int rtab_vec, const double* __restrict__ diag,       \
+      const Affine af, const int flags, const T cx, const T ch, const T cb,                     \
+      const T* __restrict__ X, const T* __restrict__ B2, const T* __restrict__ B3,              \
+      void* __restrict__ OUT, const Layout lay, const int64_t out_ld,                           \
+      double* __restrict__ dot_part, const T* __restrict__ Pr, const int64_t ngroups
+#define PF_CHAIN_PASS                                                                       \
+  P, states, binom, rtab, rtab_vec, diag, af, flags, cx, ch, cb, X, B2, B3, OUT, lay, out_ld, \
+      dot_part, Pr, ngroups
+
+template <typename T, int V, int GM, bool TBL, int NBW>
+__global__ __launch_bounds__(256) void chain_step_kernel(PF_CHAIN_ARGS) {
+  chain_step_body<T, V, GM, TBL, NBW>(PF_CHAIN_PASS);
+}
+
+// The same step held to 8 waves per SIMD (at most 64 VGPRs): more rows in
+// flight per CU to hide the neighbour-gather latency, at the price of fewer
+// gathers in flight per wave.
+template <typename T, int V, int GM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void
+chain_step_kernel_occ(PF_CHAIN_ARGS) {
+  chain_step_body<T, V, GM, true, 1>(PF_CHAIN_PASS);
+}
+#undef PF_CHAIN_ARGS
+#undef PF_CHAIN_PASS
+
 // ------------------------------------------------------ Quantum Sun kernel
 template <typename T, int V>
 __global__ __launch_bounds__(256) void qsun_step_kernel(
@@ -340,7 +367,8 @@ static bool valid_vec(int dtype, int s, int V) {
   return s / V <= 256;
 }
 
-Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt, int pb, int wide, int shfl) {
+Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt, int pb, int wide, int shfl,
+                          int occ) {
   Layout l{};
   if (V == 0 || !valid_vec(dtype, s, V)) {
     if (dtype == PF_F64)
@@ -358,6 +386,7 @@ Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt, int pb, int 
   l.pblocks = pb;
   l.wide = wide;
   l.shfl = shfl;
+  l.occ = occ;
   return l;
 }
 
@@ -378,9 +407,10 @@ Layout tuned_layout(pf_op* op, int dtype, int s) {
   if (op->force.pb >= 0) c.pb = op->force.pb;
   if (op->force.wide >= 0) c.wide = op->force.wide;
   if (op->force.shfl >= 0) c.shfl = op->force.shfl;
+  if (op->force.occ >= 0) c.occ = op->force.occ;
   return interleaved_layout(dtype, s, c.V, c.xcd < 0 ? 1 : c.xcd, c.nt < 0 ? 1 : c.nt,
                             c.pb < 0 ? default_pblocks(op) : c.pb, c.wide < 0 ? 0 : c.wide,
-                            c.shfl < 0 ? 1 : c.shfl);
+                            c.shfl < 0 ? 1 : c.shfl, c.occ < 0 ? 0 : c.occ);
 }
 
 Layout colmajor_layout(int64_t ld) {
@@ -428,9 +458,14 @@ static void launch_typed(pf_op* op, int flags, const Layout& lay, int ncols_grid
     const int rvec = (int)(op->rtab_bytes / 16);
     const size_t lds = tbl ? op->rtab_bytes : 0;
 #define PF_CHAIN_LAUNCH(GMV, TB, NW)                                                           \
-  hipLaunchKernelGGL((chain_step_kernel<T, V, GMV, TB, NW>), grid, block, lds, st, op->chain,  \
-                     op->states, op->binom, rt, rvec, op->diag, af, flags, (T)cx, (T)ch, (T)cb, \
-                     Xt, B2t, B3t, out, lay, out_ld, dot_part, Pt, ngroups)
+  if (TB && NW == 1 && lay.occ)                                                                \
+    hipLaunchKernelGGL((chain_step_kernel_occ<T, V, GMV>), grid, block, lds, st, op->chain,    \
+                       op->states, op->binom, rt, rvec, op->diag, af, flags, (T)cx, (T)ch,     \
+                       (T)cb, Xt, B2t, B3t, out, lay, out_ld, dot_part, Pt, ngroups);          \
+  else                                                                                         \
+    hipLaunchKernelGGL((chain_step_kernel<T, V, GMV, TB, NW>), grid, block, lds, st, op->chain,  \
+                       op->states, op->binom, rt, rvec, op->diag, af, flags, (T)cx, (T)ch,     \
+                       (T)cb, Xt, B2t, B3t, out, lay, out_ld, dot_part, Pt, ngroups)
 #define PF_CHAIN_G(TB, NW)                      \
   switch (lay.G) {                              \
     case 1: PF_CHAIN_LAUNCH(1, TB, NW); break;  \
@@ -1183,6 +1218,9 @@ extern "C" int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value) {
     } else if (key == PF_TUNE_NONTEMPORAL) {
       PF_REQUIRE(value >= -1 && value <= 1, "tuning: non-temporal is -1, 0 or 1");
       op->force.nt = value;
+    } else if (key == PF_TUNE_OCCUPANCY) {
+      PF_REQUIRE(value >= -1 && value <= 1, "tuning: occupancy variant is -1, 0 or 1");
+      op->force.occ = value;
     } else if (key == PF_TUNE_GRAIN_SHUFFLE) {
       PF_REQUIRE(value >= -1 && value <= 1, "tuning: grain shuffle is -1, 0 or 1");
       op->force.shfl = value;
@@ -1233,29 +1271,30 @@ extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nste
     const int pbs[3] = {default_pblocks(op), 4, 0};
     for (int vi = 0; vi < 6; ++vi) {
       if (!valid_vec(dtype, ncols, vs[vi])) continue;
-      // second variant: wide gather batches (chains with tables) or grain
-      // reloads instead of shuffles (Quantum Sun)
-      const int nwide = ((op->kind == kChain && op->chain.tsplit > 0) || op->kind == kQsun) ? 2 : 1;
+      // kernel variants: for spin chains with rank tables the plain step, wide
+      // gather batches and the 8-waves-per-SIMD build; for the Quantum Sun
+      // grain shuffles and grain reloads
+      const bool ctab = op->kind == kChain && op->chain.tsplit > 0;
+      const int nvar = ctab ? 3 : (op->kind == kQsun ? 2 : 1);
       for (int pi = 0; pi < 3; ++pi) {
         if (pi > 0 && pbs[pi] == pbs[0]) continue;
-        for (int x = 1; x >= 0; --x) {
-          for (int w = 0; w < nwide; ++w) {
-            TuneCfg c;
-            c.V = vs[vi];
-            c.xcd = x;
-            c.nt = 1;
-            c.pb = pbs[pi];
-            c.wide = op->kind == kChain ? w : 0;
-            c.shfl = op->kind == kQsun ? 1 - w : 1;
-            cands.push_back({c, 1e300});
-          }
+        for (int w = 0; w < nvar; ++w) {
+          TuneCfg c;
+          c.V = vs[vi];
+          c.xcd = 1;
+          c.nt = 1;
+          c.pb = pbs[pi];
+          c.wide = ctab && w == 1 ? 1 : 0;
+          c.occ = ctab && w == 2 ? 1 : 0;
+          c.shfl = op->kind == kQsun ? 1 - w : 1;
+          cands.push_back({c, 1e300});
         }
       }
     }
     for (int round = 0; round < 3; ++round) {
       for (auto& cd : cands) {
         const Layout lay = interleaved_layout(dtype, ncols, cd.c.V, cd.c.xcd, cd.c.nt, cd.c.pb,
-                                              cd.c.wide, cd.c.shfl);
+                                              cd.c.wide, cd.c.shfl, cd.c.occ);
         for (int w = 0; w < 2; ++w)
           launch_step(op, dtype, kHasX | kHasH | kHasB3, lay, ncols, af, 0.5, 2.0, -1.0, b[0],
                       b[1], b[2], b[3], 0, st);

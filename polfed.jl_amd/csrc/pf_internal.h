@@ -141,6 +141,7 @@ struct Layout {
   int pblocks;     // persistent grid: blocks per CU (0: one block per row group)
   int wide;        // 1: twice the neighbour gathers in flight per batch (spin chains)
   int shfl;        // 1: Quantum-Sun grain block by cross-lane shuffles (else reloads)
+  int occ;         // 1: spin-chain kernel held to 8 waves per SIMD (64 VGPRs)
 };
 
 // Launch configuration of the row-interleaved step kernel for one (dtype, s).
@@ -151,6 +152,7 @@ struct TuneCfg {
   int pb = -1;  // persistent blocks per CU (-1 = default)
   int wide = -1;  // wide gather batches (-1 = default)
   int shfl = -1;  // Quantum-Sun grain shuffles (-1 = default)
+  int occ = -1;   // spin-chain 8-waves-per-SIMD variant (-1 = default)
 };
 
 }  // namespace pf
@@ -201,7 +203,7 @@ void launch_step(pf_op* op, int dtype, int flags, const Layout& lay, int ncols, 
                  const void* P = nullptr, int dot_blocks = 0);
 
 Layout interleaved_layout(int dtype, int s, int V = 0, int xcd = 1, int nt = 1, int pb = 0,
-                          int wide = 0, int shfl = 1);
+                          int wide = 0, int shfl = 1, int occ = 0);
 Layout tuned_layout(pf_op* op, int dtype, int s);
 Layout colmajor_layout(int64_t ld);
 

@@ -132,7 +132,8 @@ class Operator:
 
     def set_tuning(self, xcd: int | None = None, vec: int | None = None, nt: int | None = None,
                    pblocks: int | None = None, wide: int | None = None,
-                   grain_shuffle: int | None = None, reset: bool = False):
+                   grain_shuffle: int | None = None, occupancy: int | None = None,
+                   reset: bool = False):
         """Force launch knobs of the row-interleaved kernels (see pf_op_set_tuning)."""
         if reset:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_RESET, 0))
@@ -143,6 +144,8 @@ class Operator:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_WIDE, int(wide)))
         if grain_shuffle is not None:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_GRAIN_SHUFFLE, int(grain_shuffle)))
+        if occupancy is not None:
+            check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_OCCUPANCY, int(occupancy)))
         if xcd is not None:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_XCD_REMAP, int(xcd)))
         if vec is not None:

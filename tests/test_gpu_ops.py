@@ -108,8 +108,9 @@ def test_long_chain_without_rank_tables(dev):
     assert _rel(dev.to_host(op.filter(c, dev.to_device(X))), oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)) < 1e-12
 
 
-@pytest.mark.parametrize("pblocks,wide,vec", [(0, 0, 2), (1, 1, 1), (8, 1, 4), (8, 0, 8)])
-def test_filter_persistent_grid_settings_agree(dev, pblocks, wide, vec):
+@pytest.mark.parametrize("pblocks,wide,vec,occ", [(0, 0, 2, 0), (1, 1, 1, 0), (8, 1, 4, 0),
+                                                 (8, 0, 8, 0), (8, 0, 2, 1), (0, 0, 4, 1)])
+def test_filter_persistent_grid_settings_agree(dev, pblocks, wide, vec, occ):
     """The persistent-grid, gather-batch and vector-width knobs change only
     how rows and neighbours are scheduled, not the result."""
     op, H = _chain(dev, *CHAINS[2][1:])
@@ -119,7 +120,7 @@ def test_filter_persistent_grid_settings_agree(dev, pblocks, wide, vec):
     Ht = (H - b * sp.identity(H.shape[0])) / a
     c = oc.filter_coefficients(-0.2, 150)
     X = np.random.default_rng(7).standard_normal((H.shape[0], 8))
-    op.set_tuning(pblocks=pblocks, wide=wide, vec=vec)
+    op.set_tuning(pblocks=pblocks, wide=wide, vec=vec, occupancy=occ)
     Y = dev.to_host(op.filter(c, dev.to_device(X)))
     assert _rel(Y, oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)) < 1e-12
 

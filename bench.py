@@ -147,7 +147,13 @@ def filter_only(args):
     op, x0 = build_workload(dev)
     op.set_rescale(-17.2, 13.5)
     dt = args.dtype
-    op.ensure_tuned(8, dt)
+    if args.layout:  # a fixed launch configuration (no autotuner trials in the profile)
+        V, pb, wide, part = (int(v) for v in args.layout.split(","))
+        op.set_tuning(vec=V, pblocks=pb, wide=wide, row_partition=part)
+    else:
+        op.ensure_tuned(8, dt)
+    if args.row_partition >= 0:  # force the row order over the autotuned choice
+        op.set_tuning(row_partition=args.row_partition)
     c = np.cos(np.arange(201) * 0.3) * 0.01
     y = torch.empty_like(x0)
     reps = max(1, args.filter_only // 201)
@@ -161,7 +167,8 @@ def filter_only(args):
     torch.cuda.synchronize()
     us = e0.elapsed_time(e1) * 1e3 / (reps * 201)
     esz = 8 if dt == "f64" else 4
-    print(json.dumps({"mode": "filter-only", "kernel": kernel_name(op, dt), "dtype": dt,
+    print(json.dumps({"mode": "filter-only", "kernel": kernel_name(op, dt),
+                      "layout": args.layout or "autotuned", "dtype": dt,
                       "D": op.D, "s": 8, "steps": reps * 201, "us_per_step": us,
                       "algorithmic_GBps": 4 * 8 * op.D * esz / (us * 1e-6) / 1e9}), flush=True)
 
@@ -481,6 +488,12 @@ def main():
                     help="profiling driver: time about STEPS filter steps only")
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"],
                     help="filter arithmetic for --filter-only")
+    ap.add_argument("--layout", default="", metavar="V,PBLOCKS,WIDE,PARTITION",
+                    help="--filter-only: fixed step-kernel launch configuration instead of "
+                         "the autotuner's (e.g. 2,8,0,1)")
+    ap.add_argument("--row-partition", type=int, default=-1, choices=[-1, 0, 1],
+                    help="--filter-only: spin-chain row order (-1 autotuned, 0 reference, "
+                         "1 XCD partition)")
     ap.add_argument("--reorth-only", type=int, default=0, metavar="CALLS",
                     help="profiling driver: time CALLS CGS2 reorthogonalizations")
     ap.add_argument("--workload", default="xxz-l20",

@@ -158,7 +158,8 @@ enum {
   PF_TUNE_PBLOCKS = 5,
   PF_TUNE_WIDE = 6,
   PF_TUNE_GRAIN_SHUFFLE = 7,
-  PF_TUNE_OCCUPANCY = 8
+  PF_TUNE_OCCUPANCY = 8,
+  PF_TUNE_ROW_PARTITION = 9
 };
 
 /* Force a launch knob of the row-interleaved step kernels for this operator:
@@ -171,7 +172,11 @@ enum {
  * batch in the spin-chain kernel with rank tables), PF_TUNE_GRAIN_SHUFFLE
  * (-1 default, 0/1: Quantum-Sun grain block by cross-lane shuffles or by
  * reloads), PF_TUNE_OCCUPANCY (-1 default, 0/1: spin-chain kernel built for
- * 8 waves per SIMD),
+ * 8 waves per SIMD), PF_TUNE_ROW_PARTITION (-1 default, 0/1: spin chains
+ * with rank tables run pf_filter / pf_kpm_moments on the basis reordered so
+ * that the row range each XCD owns gathers mostly its own rows; inputs and
+ * outputs stay in the reference order and filter results are bit-identical;
+ * the autotuner tries both orders),
  * PF_TUNE_RESET (drop forced and autotuned settings).  No reference
  * counterpart: launch tuning of this library. */
 int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value);

@@ -34,6 +34,7 @@ PF_TUNE_PBLOCKS = 5
 PF_TUNE_WIDE = 6
 PF_TUNE_GRAIN_SHUFFLE = 7
 PF_TUNE_OCCUPANCY = 8
+PF_TUNE_ROW_PARTITION = 9
 
 PF_BATCH_GRAPH = 1
 PF_BATCH_GRAPH_STEPS = 2

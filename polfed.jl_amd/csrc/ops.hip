@@ -39,12 +39,14 @@ __device__ __forceinline__ void step_load(int flags, int64_t off, const T* __res
 }
 
 // out = cx*X + ch*h + cb*b3, store (interleaved T, or FP64 column-major with
-// kOutCM), optional KPM dot <P, out>.
+// kOutCM at row operm[row], the reference rank, when the rows run in the
+// partition order), optional KPM dot <P, out>.
 template <typename T, int V>
 __device__ __forceinline__ void step_store(int flags, int64_t row, int64_t off, int out_col0,
                                            T cx, T ch, T cb, typename VT<T, V>::t h,
                                            typename VT<T, V>::t xv, typename VT<T, V>::t b3v,
                                            void* __restrict__ OUT, int64_t out_ld,
+                                           const uint32_t* __restrict__ operm,
                                            const T* __restrict__ Pr, double& dacc) {
   typedef typename VT<T, V>::t vt;
   vt o = (vt)(T)0;
@@ -54,8 +56,9 @@ __device__ __forceinline__ void step_store(int flags, int64_t row, int64_t off, 
   if (flags & kHasB3) o += cb * b3v;
   if (flags & kOutCM) {
     double* Y = reinterpret_cast<double*>(OUT);
+    const int64_t yr = operm ? (int64_t)operm[row] : row;
 #pragma unroll
-    for (int v = 0; v < V; ++v) Y[row + (int64_t)(out_col0 + v) * out_ld] = (double)vget<T, V>(o, v);
+    for (int v = 0; v < V; ++v) Y[yr + (int64_t)(out_col0 + v) * out_ld] = (double)vget<T, V>(o, v);
   } else if (nt) {
     vstore_nt<T, V>(reinterpret_cast<T*>(OUT) + off, o);
   } else {
@@ -98,15 +101,20 @@ struct RankCtx {
   int m;
 };
 
-// Rank of the neighbour across bond b, or r when its two spins are parallel.
+// Rank of the neighbour across bond b, or r when its two spins are parallel
+// (also for the zero padding bonds, mask 0).  With tables the lookups are made
+// for every bond -- any L-bit value indexes both tables in bounds -- and the
+// result selected afterwards, so a batch issues all its LDS reads before one
+// wait and takes no branch.
 template <bool TBL>
 __device__ __forceinline__ uint32_t one_rank(const RankCtx& R, int b, uint32_t st, uint32_t r) {
   const uint32_t mk = R.M[b];
-  if (popc32(st & mk) != 1) return r;
   if constexpr (TBL) {
     const uint32_t x = st ^ mk;
-    return (uint32_t)R.TA[x & R.maskA] + R.TB[x >> R.m];
+    const uint32_t nr = (uint32_t)R.TA[x & R.maskA] + R.TB[x >> R.m];
+    return popc32(st & mk) == 1 ? nr : r;
   } else {
+    if (popc32(st & mk) != 1) return r;
     return neighbor_rank(st, r, ctz32(mk), 31 - __clz(mk), R.C);
   }
 }
@@ -114,23 +122,18 @@ __device__ __forceinline__ uint32_t one_rank(const RankCtx& R, int b, uint32_t s
 // Ranks of the neighbours of bonds b0 + j, j < NBAT, as seen by every lane of
 // the row group: with GM in {1,2,4,8} lane g computes the bonds j = g + GM*u
 // and the group shares them; GM = 0 (any other group size) computes all NBAT.
+// The bond list is padded with zero bonds to whole batches.
 template <int GM, int NBAT, bool TBL>
-__device__ __forceinline__ void bond_ranks(const RankCtx& R, int nb, int b0, uint32_t st,
-                                           uint32_t r, int g, uint32_t (&out)[NBAT]) {
+__device__ __forceinline__ void bond_ranks(const RankCtx& R, int b0, uint32_t st, uint32_t r,
+                                           int g, uint32_t (&out)[NBAT]) {
   if constexpr (GM == 0) {
 #pragma unroll
-    for (int j = 0; j < NBAT; ++j) {
-      const int b = b0 + j;
-      out[j] = b < nb ? one_rank<TBL>(R, b, st, r) : r;  // b < nb is uniform
-    }
+    for (int j = 0; j < NBAT; ++j) out[j] = one_rank<TBL>(R, b0 + j, st, r);
   } else {
     constexpr int UB = NBAT / GM;
     uint32_t mine[UB];
 #pragma unroll
-    for (int u = 0; u < UB; ++u) {
-      const int b = b0 + g + GM * u;
-      mine[u] = b < nb ? one_rank<TBL>(R, b, st, r) : r;
-    }
+    for (int u = 0; u < UB; ++u) mine[u] = one_rank<TBL>(R, b0 + g + GM * u, st, r);
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
 #pragma unroll
@@ -151,20 +154,33 @@ __device__ __forceinline__ void bond_ranks(const RankCtx& R, int nb, int b0, uin
   }
 }
 
+// Byte offset of row r (row stride rsb bytes, rsh = log2(rsb) or -1).  With
+// rank tables D < 2^24 (L <= 26), so one full-rate 24-bit multiply serves
+// every stride; otherwise a shift or a 32-bit multiply.
+template <bool TBL>
+__device__ __forceinline__ uint32_t row_bytes(uint32_t r, uint32_t rsb, int rsh) {
+  if constexpr (TBL) return __umul24(r, rsb);
+  else return rsh >= 0 ? (r << rsh) : r * rsb;
+}
+
 // out = cx*X + ch*((H - shift)*scale) b2 + cb*b3 on rows of the Lup sector.
 // The G = s/V lanes of a row split the neighbour-rank work (GM = G) and each
 // gathers its V columns of every flippable neighbour; NBAT neighbours are
 // loaded before any is accumulated.  Block p (XCD-grouped) owns the
 // contiguous row groups [p*n/nblk, (p+1)*n/nblk): with a persistent grid the
-// rank tables are staged in LDS once per block, not once per row group.
+// rank tables are staged in LDS once per block, not once per row group.  A
+// parallel-spin bond's gather is skipped (masked lanes); gathering a zero row
+// instead removes the masking instructions but costs more memory-pipeline
+// work than it saves.
 template <typename T, int V, int GM, bool TBL, int NBW>
 __device__ __forceinline__ void chain_step_body(
     const ChainParams& P, const uint32_t* __restrict__ states, const uint32_t* __restrict__ binom,
     const uint4* __restrict__ rtab, const int rtab_vec, const double* __restrict__ diag,
     const Affine af, const int flags, const T cx, const T ch, const T cb,
     const T* __restrict__ X, const T* __restrict__ B2, const T* __restrict__ B3,
-    void* __restrict__ OUT, const Layout& lay, const int64_t out_ld, double* __restrict__ dot_part,
-    const T* __restrict__ Pr, const int64_t ngroups) {
+    void* __restrict__ OUT, const Layout& lay, const int64_t out_ld,
+    const uint32_t* __restrict__ operm, double* __restrict__ dot_part, const T* __restrict__ Pr,
+    const int64_t ngroups) {
   typedef typename VT<T, V>::t vt;
   constexpr int GE = GM < 1 ? 1 : GM;
   constexpr int NB0 = ((V * (int)sizeof(T) <= 32) ? 8 : 4) * NBW;  // gathers in flight
@@ -187,9 +203,12 @@ __device__ __forceinline__ void chain_step_body(
     for (int i = threadIdx.x; i < 32 * kRankT; i += blockDim.x) sC[i] = binom[i];
     R.C = sC;
   }
-  for (int i = threadIdx.x; i < P.nb; i += blockDim.x) {
-    sW[i] = (T)(P.bval[i] * af.scale);
-    sM[i] = (1u << P.lo[i]) | (1u << P.hi[i]);
+  // bonds padded with zero bonds (mask 0, weight 0) to whole batches of NBAT
+  const int nbp = (P.nb + NBAT - 1) / NBAT * NBAT;  // <= kMaxBonds (a multiple of NBAT)
+  for (int i = threadIdx.x; i < nbp; i += blockDim.x) {
+    const bool real = i < P.nb;
+    sW[i] = real ? (T)(P.bval[i] * af.scale) : (T)0;
+    sM[i] = real ? (1u << P.lo[i]) | (1u << P.hi[i]) : 0u;
   }
   __syncthreads();
 
@@ -239,29 +258,27 @@ __device__ __forceinline__ void chain_step_body(
     if (use_h) {
       const uint32_t r32 = (uint32_t)row;
       vt tot = (vt)(T)0;
-      for (int b0 = 0; b0 < P.nb; b0 += NBAT) {
+      for (int b0 = 0; b0 < nbp; b0 += NBAT) {
         uint32_t rk[NBAT];
-        bond_ranks<GM, NBAT, TBL>(R, P.nb, b0, st, r32, g, rk);
+        bond_ranks<GM, NBAT, TBL>(R, b0, st, r32, g, rk);
         vt x[NBAT];
 #pragma unroll
         for (int j = 0; j < NBAT; ++j) {
           x[j] = (vt)(T)0;
           // 32-bit byte offset from the block base (D*rs*sizeof(T) < 2^32 is
           // checked on the host)
-          const uint32_t o = (rsh >= 0 ? (rk[j] << rsh) : rk[j] * rsb) + cofs_b;
+          const uint32_t o = row_bytes<TBL>(rk[j], rsb, rsh) + cofs_b;
           if (rk[j] != r32) x[j] = *reinterpret_cast<const vt*>(b2b + o);
         }
 #pragma unroll
-        for (int j = 0; j < NBAT; ++j) {
-          const int b = b0 + j;
-          if (b < P.nb) tot += sW[b] * x[j];
-        }
+        for (int j = 0; j < NBAT; ++j) tot += sW[b0 + j] * x[j];
       }
       const T d = (T)((drow - af.shift) * af.scale);
-      const uint32_t o = (rsh >= 0 ? (r32 << rsh) : r32 * rsb) + cofs_b;
+      const uint32_t o = row_bytes<TBL>(r32, rsb, rsh) + cofs_b;
       h = d * *reinterpret_cast<const vt*>(b2b + o) + tot;
     }
-    step_store<T, V>(flags, row, off, out_col0, cx, ch, cb, h, xv, b3v, OUT, out_ld, Pr, dacc);
+    step_store<T, V>(flags, row, off, out_col0, cx, ch, cb, h, xv, b3v, OUT, out_ld, operm, Pr,
+                     dacc);
   }
   if (flags & kDot) {
     const double t = block_sum_256(dacc, red);
@@ -275,10 +292,11 @@ __device__ __forceinline__ void chain_step_body(
       const Affine af, const int flags, const T cx, const T ch, const T cb,                     \
       const T* __restrict__ X, const T* __restrict__ B2, const T* __restrict__ B3,              \
       void* __restrict__ OUT, const Layout lay, const int64_t out_ld,                           \
-      double* __restrict__ dot_part, const T* __restrict__ Pr, const int64_t ngroups
+      const uint32_t* __restrict__ operm, double* __restrict__ dot_part,                        \
+      const T* __restrict__ Pr, const int64_t ngroups
 #define PF_CHAIN_PASS                                                                       \
   P, states, binom, rtab, rtab_vec, diag, af, flags, cx, ch, cb, X, B2, B3, OUT, lay, out_ld, \
-      dot_part, Pr, ngroups
+      operm, dot_part, Pr, ngroups
 
 template <typename T, int V, int GM, bool TBL, int NBW>
 __global__ __launch_bounds__(256) void chain_step_kernel(PF_CHAIN_ARGS) {
@@ -350,7 +368,8 @@ __global__ __launch_bounds__(256) void qsun_step_kernel(
     step_load<T, V>(flags, off, X, B3, xv, b3v);
     vt h = (vt)(T)0;
     if (flags & kHasH) h = qsun_hrow_dg<T, V>(q, (uint32_t)row, sG, G_);
-    step_store<T, V>(flags, row, off, out_col0, cx, ch, cb, h, xv, b3v, OUT, out_ld, Pr, dacc);
+    step_store<T, V>(flags, row, off, out_col0, cx, ch, cb, h, xv, b3v, OUT, out_ld, nullptr, Pr,
+                     dacc);
   }
   if (flags & kDot) {
     const double t = block_sum_256(dacc, red);
@@ -368,7 +387,7 @@ static bool valid_vec(int dtype, int s, int V) {
 }
 
 Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt, int pb, int wide, int shfl,
-                          int occ) {
+                          int occ, int part) {
   Layout l{};
   if (V == 0 || !valid_vec(dtype, s, V)) {
     if (dtype == PF_F64)
@@ -387,6 +406,7 @@ Layout interleaved_layout(int dtype, int s, int V, int xcd, int nt, int pb, int 
   l.wide = wide;
   l.shfl = shfl;
   l.occ = occ;
+  l.part = part;
   return l;
 }
 
@@ -408,9 +428,11 @@ Layout tuned_layout(pf_op* op, int dtype, int s) {
   if (op->force.wide >= 0) c.wide = op->force.wide;
   if (op->force.shfl >= 0) c.shfl = op->force.shfl;
   if (op->force.occ >= 0) c.occ = op->force.occ;
+  if (op->force.part >= 0) c.part = op->force.part;
   return interleaved_layout(dtype, s, c.V, c.xcd < 0 ? 1 : c.xcd, c.nt < 0 ? 1 : c.nt,
                             c.pb < 0 ? default_pblocks(op) : c.pb, c.wide < 0 ? 0 : c.wide,
-                            c.shfl < 0 ? 1 : c.shfl, c.occ < 0 ? 0 : c.occ);
+                            c.shfl < 0 ? 1 : c.shfl, c.occ < 0 ? 0 : c.occ,
+                            c.part < 0 ? 0 : c.part);
 }
 
 Layout colmajor_layout(int64_t ld) {
@@ -454,18 +476,23 @@ static void launch_typed(pf_op* op, int flags, const Layout& lay, int ncols_grid
   const T* Pt = reinterpret_cast<const T*>(P);
   if (op->kind == kChain) {
     const bool tbl = op->chain.tsplit > 0 && op->rtab;
-    const uint4* rt = reinterpret_cast<const uint4*>(op->rtab);
+    // drivers run in the partition order when it is built; the hooks never do
+    const bool part = (flags & kRowPart) && op->pstates;
+    const uint32_t* sts = part ? op->pstates : op->states;
+    const double* dg = part ? op->pdiag : op->diag;
+    const uint32_t* operm = part ? op->pperm : nullptr;
+    const uint4* rt = reinterpret_cast<const uint4*>(part ? op->prtab : op->rtab);
     const int rvec = (int)(op->rtab_bytes / 16);
     const size_t lds = tbl ? op->rtab_bytes : 0;
 #define PF_CHAIN_LAUNCH(GMV, TB, NW)                                                           \
   if (TB && NW == 1 && lay.occ)                                                                \
     hipLaunchKernelGGL((chain_step_kernel_occ<T, V, GMV>), grid, block, lds, st, op->chain,    \
-                       op->states, op->binom, rt, rvec, op->diag, af, flags, (T)cx, (T)ch,     \
-                       (T)cb, Xt, B2t, B3t, out, lay, out_ld, dot_part, Pt, ngroups);          \
+                       sts, op->binom, rt, rvec, dg, af, flags, (T)cx, (T)ch, (T)cb, Xt, B2t,  \
+                       B3t, out, lay, out_ld, operm, dot_part, Pt, ngroups);                   \
   else                                                                                         \
     hipLaunchKernelGGL((chain_step_kernel<T, V, GMV, TB, NW>), grid, block, lds, st, op->chain,  \
-                       op->states, op->binom, rt, rvec, op->diag, af, flags, (T)cx, (T)ch,     \
-                       (T)cb, Xt, B2t, B3t, out, lay, out_ld, dot_part, Pt, ngroups)
+                       sts, op->binom, rt, rvec, dg, af, flags, (T)cx, (T)ch, (T)cb, Xt, B2t,  \
+                       B3t, out, lay, out_ld, operm, dot_part, Pt, ngroups)
 #define PF_CHAIN_G(TB, NW)                      \
   switch (lay.G) {                              \
     case 1: PF_CHAIN_LAUNCH(1, TB, NW); break;  \
@@ -521,11 +548,14 @@ void launch_step(pf_op* op, int dtype, int flags, const Layout& lay, int ncols, 
 }
 
 // ------------------------------------------------------------- pack kernel
-// dst[r*s + c] = (T)(src[r + c*ld] * scale[c]) through a 64-row LDS tile.
+// dst[r*s + c] = (T)(src[perm[r] + c*ld] * scale[c]) through a 64-row LDS
+// tile (perm null: the identity).  A partition permutation keeps segments of
+// consecutive reference ranks together, so the column reads stay in runs.
 template <typename T>
 __global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ src, int64_t ld,
                                                    T* __restrict__ dst, int64_t D, int s,
-                                                   const double* __restrict__ scale) {
+                                                   const double* __restrict__ scale,
+                                                   const uint32_t* __restrict__ perm) {
   extern __shared__ double tile[];  // 64 x (s+1)
   const int64_t r0 = (int64_t)blockIdx.x * 64;
   const int n = 64 * s;
@@ -533,7 +563,7 @@ __global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ sr
     const int c = idx / 64, rr = idx - c * 64;
     const int64_t r = r0 + rr;
     double v = 0.0;
-    if (r < D) v = src[r + (int64_t)c * ld] * (scale ? scale[c] : 1.0);
+    if (r < D) v = src[(perm ? (int64_t)perm[r] : r) + (int64_t)c * ld] * (scale ? scale[c] : 1.0);
     tile[rr * (s + 1) + c] = v;
   }
   __syncthreads();
@@ -545,16 +575,16 @@ __global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ sr
 }
 
 void pack_block(int dtype, const double* src, int64_t ld, void* dst, int64_t D, int s,
-                const double* scale, hipStream_t st) {
+                const double* scale, hipStream_t st, const uint32_t* perm) {
   PF_REQUIRE(s >= 1 && s <= 64, "pack: 1 <= s <= 64");
   const int64_t nb = (D + 63) / 64;
   const size_t lds = (size_t)64 * (s + 1) * sizeof(double);
   if (dtype == PF_F64)
     hipLaunchKernelGGL(pack_kernel<double>, dim3((unsigned)nb), dim3(256), lds, st, src, ld,
-                       reinterpret_cast<double*>(dst), D, s, scale);
+                       reinterpret_cast<double*>(dst), D, s, scale, perm);
   else
     hipLaunchKernelGGL(pack_kernel<float>, dim3((unsigned)nb), dim3(256), lds, st, src, ld,
-                       reinterpret_cast<float*>(dst), D, s, scale);
+                       reinterpret_cast<float*>(dst), D, s, scale, perm);
   PF_HIP(hipGetLastError());
 }
 
@@ -710,6 +740,100 @@ __global__ void kpm_reduce_kernel(const double* __restrict__ part, int stride, i
   mu[n] += t;
 }
 
+// Order of the high-bit segments (b = state >> m) for the XCD partition:
+// reverse Cuthill-McKee over the graph that joins segment b to every segment
+// its rows reach through one bond.  The drivers give each XCD a contiguous
+// range of rows; in this order most neighbour rows of a range fall in the
+// range itself (its own L2) instead of being spread over all XCDs as in the
+// colex order.  Returns the valid segments (0 <= Lup - |b| <= m) in order.
+static std::vector<uint32_t> rcm_segment_order(const ChainParams& P, int m) {
+  const uint32_t nB = 1u << (P.L - m);
+  std::vector<int> id(nB, -1);
+  std::vector<uint32_t> seg;
+  for (uint32_t b = 0; b < nB; ++b) {
+    const int na = P.Lup - popc32(b);
+    if (na >= 0 && na <= m) {
+      id[b] = (int)seg.size();
+      seg.push_back(b);
+    }
+  }
+  const int n = (int)seg.size();
+  std::vector<std::vector<int>> adj(n);
+  for (int i = 0; i < n; ++i) {
+    const uint32_t b = seg[i];
+    for (int k = 0; k < P.nb; ++k) {
+      const int lo = P.lo[k], hi = P.hi[k];
+      if (hi < m) continue;  // both sites in the low bits: the segment stays
+      uint32_t nbs;
+      if (lo >= m) {
+        const uint32_t mk = (1u << (lo - m)) | (1u << (hi - m));
+        if ((b & mk) == 0 || (b & mk) == mk) continue;
+        nbs = b ^ mk;
+      } else {
+        nbs = b ^ (1u << (hi - m));  // the low site flips with it
+      }
+      if (id[nbs] >= 0) adj[i].push_back(id[nbs]);
+    }
+    std::sort(adj[i].begin(), adj[i].end());
+    adj[i].erase(std::unique(adj[i].begin(), adj[i].end()), adj[i].end());
+  }
+  std::vector<int> deg(n);
+  for (int i = 0; i < n; ++i) deg[i] = (int)adj[i].size();
+  for (auto& a : adj)  // Cuthill-McKee visits neighbours by ascending degree
+    std::sort(a.begin(), a.end(),
+              [&](int x, int y) { return deg[x] != deg[y] ? deg[x] < deg[y] : x < y; });
+  std::vector<char> done(n, 0);
+  std::vector<int> seen(n, 0), lvl(n, 0);
+  int stamp = 0;
+  // breadth-first order from s over the nodes not yet placed; returns the depth
+  auto bfs = [&](int s, std::vector<int>& out) {
+    ++stamp;
+    out.clear();
+    out.push_back(s);
+    seen[s] = stamp;
+    lvl[s] = 0;
+    for (size_t h = 0; h < out.size(); ++h) {
+      const int u = out[h];
+      for (int v : adj[u])
+        if (!done[v] && seen[v] != stamp) {
+          seen[v] = stamp;
+          lvl[v] = lvl[u] + 1;
+          out.push_back(v);
+        }
+    }
+    return lvl[out.back()];
+  };
+  std::vector<int> order, comp, trial;
+  order.reserve(n);
+  for (int s0 = 0; s0 < n; ++s0) {
+    if (done[s0]) continue;
+    // pseudo-peripheral start: move to a least-degree node of the deepest
+    // level while that makes the breadth-first depth grow
+    int s = s0;
+    int depth = bfs(s, comp);
+    for (int it = 0; it < 16; ++it) {
+      const int last = lvl[comp.back()];
+      int cand = comp.back();
+      for (int i = (int)comp.size() - 1; i >= 0 && lvl[comp[i]] == last; --i)
+        if (deg[comp[i]] < deg[cand]) cand = comp[i];
+      const int d2 = bfs(cand, trial);
+      if (d2 <= depth) break;
+      depth = d2;
+      s = cand;
+      comp.swap(trial);
+    }
+    bfs(s, comp);
+    for (int u : comp) {
+      done[u] = 1;
+      order.push_back(u);
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  std::vector<uint32_t> out(n);
+  for (int i = 0; i < n; ++i) out[i] = seg[order[i]];
+  return out;
+}
+
 }  // namespace pf
 
 using namespace pf;
@@ -795,7 +919,7 @@ extern "C" int pf_op_create_spin_chain(pf_handle* h, int32_t L, int32_t Lup, int
     // and TB[b] = sum_t' C(m + q_t', nA + t' + 1) over the set bits q of b,
     // nA = Lup - |b| (the particles of a come first in the combinadic sum).
     std::vector<unsigned char> rtab;
-    if (L <= kMaxTableL) {
+    if (L <= kMaxTableL && D < ((uint64_t)1 << 24)) {  // ranks fit 24-bit offsets
       const int m = (L + 1) / 2;
       const uint32_t nA = 1u << m, nB = 1u << (L - m);
       const size_t bytes = ((size_t)nA * 2 + (size_t)nB * 4 + 15) & ~(size_t)15;
@@ -825,11 +949,54 @@ extern "C" int pf_op_create_spin_chain(pf_handle* h, int32_t L, int32_t Lup, int
     } else {
       P.tsplit = 0;
     }
+    // The XCD partition order: the same segments (rows of equal high bits, in
+    // colex order inside) placed in rcm_segment_order; TA is unchanged.
+    std::vector<unsigned char> prtab;
+    std::vector<uint32_t> pstates, pperm;
+    if (P.tsplit > 0 && P.nb > 0) {
+      const int m = P.tsplit;
+      const uint32_t nA = 1u << m, nB = 1u << (L - m);
+      const uint16_t* TA = reinterpret_cast<const uint16_t*>(rtab.data());
+      prtab = rtab;
+      uint32_t* TBp = reinterpret_cast<uint32_t*>(prtab.data() + (size_t)nA * 2);
+      std::fill(TBp, TBp + nB, 0u);
+      uint64_t off = 0;
+      for (const uint32_t b : rcm_segment_order(P, m)) {
+        TBp[b] = (uint32_t)off;
+        off += C[m * kRankT + (Lup - popc32(b))];  // C(m, set bits of the low part)
+      }
+      pstates.assign(D, 0u);
+      pperm.assign(D, ~0u);
+      bool ok = off == D;
+      for (uint64_t r = 0; ok && r < D; ++r) {
+        const uint32_t st = states[r];
+        const uint64_t i = (uint64_t)TA[st & (nA - 1)] + TBp[st >> m];
+        if (i >= D || pperm[i] != ~0u) {
+          ok = false;
+          break;
+        }
+        pstates[i] = st;
+        pperm[i] = (uint32_t)r;
+      }
+      if (!ok) {
+        delete op;
+        throw Error{PF_ERR_INVALID, "spin chain: partition order check failed"};
+      }
+    }
     try {
       if (!rtab.empty()) {
         PF_HIP(hipMalloc(&op->rtab, rtab.size()));
         PF_HIP(hipMemcpy(op->rtab, rtab.data(), rtab.size(), hipMemcpyHostToDevice));
         op->rtab_bytes = rtab.size();
+      }
+      if (!pstates.empty()) {
+        PF_HIP(hipMalloc(&op->prtab, prtab.size()));
+        PF_HIP(hipMalloc(&op->pstates, D * sizeof(uint32_t)));
+        PF_HIP(hipMalloc(&op->pperm, D * sizeof(uint32_t)));
+        PF_HIP(hipMalloc(&op->pdiag, D * sizeof(double)));
+        PF_HIP(hipMemcpy(op->prtab, prtab.data(), prtab.size(), hipMemcpyHostToDevice));
+        PF_HIP(hipMemcpy(op->pstates, pstates.data(), D * sizeof(uint32_t), hipMemcpyHostToDevice));
+        PF_HIP(hipMemcpy(op->pperm, pperm.data(), D * sizeof(uint32_t), hipMemcpyHostToDevice));
       }
       PF_HIP(hipMalloc(&op->states, D * sizeof(uint32_t)));
       PF_HIP(hipMalloc(&op->binom, C.size() * sizeof(uint32_t)));
@@ -855,6 +1022,11 @@ extern "C" int pf_op_create_spin_chain(pf_handle* h, int32_t L, int32_t Lup, int
       hipLaunchKernelGGL(chain_diag_kernel, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, 0,
                          op->states, (int64_t)D, L, nbonds, dfields, dbi, dbj, djd, op->diag);
       PF_HIP(hipGetLastError());
+      if (op->pstates) {  // the same per-state values, in partition order
+        hipLaunchKernelGGL(chain_diag_kernel, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, 0,
+                           op->pstates, (int64_t)D, L, nbonds, dfields, dbi, dbj, djd, op->pdiag);
+        PF_HIP(hipGetLastError());
+      }
       PF_HIP(hipDeviceSynchronize());
       (void)hipFree(dfields);
       (void)hipFree(dbi);
@@ -936,6 +1108,10 @@ extern "C" int pf_op_destroy(pf_op* op) {
   if (op->binom) (void)hipFree(op->binom);
   if (op->diag) (void)hipFree(op->diag);
   if (op->rtab) (void)hipFree(op->rtab);
+  if (op->pstates) (void)hipFree(op->pstates);
+  if (op->pdiag) (void)hipFree(op->pdiag);
+  if (op->prtab) (void)hipFree(op->prtab);
+  if (op->pperm) (void)hipFree(op->pperm);
   if (op->qdiag) (void)hipFree(op->qdiag);
   op->fbuf.release();
   op->kpart.release();
@@ -1075,6 +1251,14 @@ static void poly_phi(int type, double& phi0, double& phi1) {
 namespace pf {
 // Chunk width for the row-interleaved drivers (columns are independent).
 static int chunk_cols(int dtype) { return dtype == PF_F64 ? 32 : 64; }
+// Flag of the driver launches: the partition row order when built and chosen.
+static int driver_part(const pf_op* op, const Layout& lay) {
+  return (op->pstates && lay.part) ? kRowPart : 0;
+}
+// Reference rank of each partition row, or null for the reference order.
+static const uint32_t* driver_perm(const pf_op* op, const Layout& lay) {
+  return driver_part(op, lay) ? op->pperm : nullptr;
+}
 }  // namespace pf
 
 extern "C" int pf_filter(pf_op* op, int32_t dtype, int32_t polytype, const double* coeffs,
@@ -1100,14 +1284,15 @@ extern "C" int pf_filter(pf_op* op, int32_t dtype, int32_t polytype, const doubl
       char* base = reinterpret_cast<char*>(op->fbuf.get(4 * blk));
       void* Xp = base;
       void* b[3] = {base + blk, base + 2 * blk, base + 3 * blk};
-      pack_block(dtype, X + (int64_t)c0 * ldx, ldx, Xp, op->D, s, nullptr, st);
+      pack_block(dtype, X + (int64_t)c0 * ldx, ldx, Xp, op->D, s, nullptr, st,
+                 driver_perm(op, lay));
       double* Yc = Y + (int64_t)c0 * ldy;
       bool z2 = true, z3 = true;  // b2 / b3 known to be zero
       for (int k = order; k >= 1; --k) {
         double alpha, beta_k1, dummy;
         poly_props(polytype, k, alpha, dummy);
         poly_props(polytype, k + 1, dummy, beta_k1);
-        int flags = kHasX;
+        int flags = kHasX | driver_part(op, lay);
         if (!z2) flags |= kHasH;
         if (!z3 && beta_k1 != 0.0) flags |= kHasB3;
         launch_step(op, dtype, flags, lay, s, af, coeffs[k], alpha, beta_k1, Xp, b[1], b[2], b[0],
@@ -1121,7 +1306,7 @@ extern "C" int pf_filter(pf_op* op, int32_t dtype, int32_t polytype, const doubl
       }
       double beta1, dummy;
       poly_props(polytype, 1, dummy, beta1);
-      int flags = kHasX | kOutCM;
+      int flags = kHasX | kOutCM | driver_part(op, lay);
       if (!z2) flags |= kHasH;
       if (!z3 && beta1 * phi0 != 0.0) flags |= kHasB3;
       launch_step(op, dtype, flags, lay, s, af, coeffs[0] * phi0, phi1, beta1 * phi0, Xp, b[1],
@@ -1171,20 +1356,22 @@ extern "C" int pf_kpm_moments(pf_op* op, int32_t dtype, int32_t N, int32_t R, co
       hipLaunchKernelGGL(colnorm2_kernel, dim3(nb_norm, s), dim3(256), 0, st, src, ldp, op->D, npart);
       hipLaunchKernelGGL(inv_sqrt_sum_kernel, dim3(1), dim3(64), 0, st, npart, nb_norm, s, cscale);
       PF_HIP(hipGetLastError());
-      pack_block(dtype, src, ldp, P, op->D, s, cscale, st);
+      // (in the partition row order the dots are the same sums, reordered)
+      const int rp = driver_part(op, lay);
+      pack_block(dtype, src, ldp, P, op->D, s, cscale, st, driver_perm(op, lay));
       // mu_0: <P, P>  (an H-free step that copies P and dots it)
-      launch_step(op, dtype, kHasX | kDot, lay, s, af, 1.0, 0.0, 0.0, P, nullptr, nullptr, t[0], 0,
-                  st, part + 0, P, dot_blocks);
+      launch_step(op, dtype, kHasX | kDot | rp, lay, s, af, 1.0, 0.0, 0.0, P, nullptr, nullptr,
+                  t[0], 0, st, part + 0, P, dot_blocks);
       // mu_1: T_1 = H~ T_0
-      launch_step(op, dtype, kHasH | kDot, lay, s, af, 0.0, 1.0, 0.0, nullptr, P, nullptr, t[1], 0,
-                  st, part + (size_t)1 * dot_blocks, P, dot_blocks);
+      launch_step(op, dtype, kHasH | kDot | rp, lay, s, af, 0.0, 1.0, 0.0, nullptr, P, nullptr,
+                  t[1], 0, st, part + (size_t)1 * dot_blocks, P, dot_blocks);
       // T_{n} = 2 H~ T_{n-1} - T_{n-2}; t[0] holds T_0 (copy), t[1] T_1
       void* tm2 = t[0];
       void* tm1 = t[1];
       void* tn = t[2];
       for (int n = 2; n < N; ++n) {
-        launch_step(op, dtype, kHasH | kHasB3 | kDot, lay, s, af, 0.0, 2.0, -1.0, nullptr, tm1,
-                    tm2, tn, 0, st, part + (size_t)n * dot_blocks, P, dot_blocks);
+        launch_step(op, dtype, kHasH | kHasB3 | kDot | rp, lay, s, af, 0.0, 2.0, -1.0, nullptr,
+                    tm1, tm2, tn, 0, st, part + (size_t)n * dot_blocks, P, dot_blocks);
         void* tmp = tm2;
         tm2 = tm1;
         tm1 = tn;
@@ -1230,6 +1417,9 @@ extern "C" int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value) {
     } else if (key == PF_TUNE_PBLOCKS) {
       PF_REQUIRE(value >= -1 && value <= 64, "tuning: persistent blocks per CU is -1..64");
       op->force.pb = value;
+    } else if (key == PF_TUNE_ROW_PARTITION) {
+      PF_REQUIRE(value >= -1 && value <= 1, "tuning: row partition is -1, 0 or 1");
+      op->force.part = value;
     } else if (key == PF_TUNE_RESET) {
       op->force = TuneCfg{};
       op->tuned.clear();
@@ -1271,11 +1461,12 @@ extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nste
     const int pbs[3] = {default_pblocks(op), 4, 0};
     for (int vi = 0; vi < 6; ++vi) {
       if (!valid_vec(dtype, ncols, vs[vi])) continue;
-      // kernel variants: for spin chains with rank tables the plain step, wide
-      // gather batches and the 8-waves-per-SIMD build; for the Quantum Sun
-      // grain shuffles and grain reloads
+      // kernel variants: for spin chains with rank tables the plain step and
+      // wide gather batches, each in the reference and the partition row
+      // order; for the Quantum Sun grain shuffles and grain reloads (the
+      // 8-waves-per-SIMD build is left to forced tuning: it never won)
       const bool ctab = op->kind == kChain && op->chain.tsplit > 0;
-      const int nvar = ctab ? 3 : (op->kind == kQsun ? 2 : 1);
+      const int nvar = ctab ? (op->pstates ? 4 : 2) : (op->kind == kQsun ? 2 : 1);
       for (int pi = 0; pi < 3; ++pi) {
         if (pi > 0 && pbs[pi] == pbs[0]) continue;
         for (int w = 0; w < nvar; ++w) {
@@ -1284,8 +1475,9 @@ extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nste
           c.xcd = 1;
           c.nt = 1;
           c.pb = pbs[pi];
-          c.wide = ctab && w == 1 ? 1 : 0;
-          c.occ = ctab && w == 2 ? 1 : 0;
+          c.wide = ctab && (w & 1) ? 1 : 0;
+          c.occ = 0;
+          c.part = ctab && (w & 2) ? 1 : 0;
           c.shfl = op->kind == kQsun ? 1 - w : 1;
           cands.push_back({c, 1e300});
         }
@@ -1294,14 +1486,14 @@ extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nste
     for (int round = 0; round < 3; ++round) {
       for (auto& cd : cands) {
         const Layout lay = interleaved_layout(dtype, ncols, cd.c.V, cd.c.xcd, cd.c.nt, cd.c.pb,
-                                              cd.c.wide, cd.c.shfl, cd.c.occ);
+                                              cd.c.wide, cd.c.shfl, cd.c.occ, cd.c.part);
+        const int fl = kHasX | kHasH | kHasB3 | driver_part(op, lay);
         for (int w = 0; w < 2; ++w)
-          launch_step(op, dtype, kHasX | kHasH | kHasB3, lay, ncols, af, 0.5, 2.0, -1.0, b[0],
-                      b[1], b[2], b[3], 0, st);
+          launch_step(op, dtype, fl, lay, ncols, af, 0.5, 2.0, -1.0, b[0], b[1], b[2], b[3], 0, st);
         PF_HIP(hipEventRecord(e0, st));
         for (int k = 0; k < nsteps; ++k)  // rotate three distinct recurrence buffers
-          launch_step(op, dtype, kHasX | kHasH | kHasB3, lay, ncols, af, 0.5, 2.0, -1.0, b[0],
-                      b[1 + k % 3], b[1 + (k + 1) % 3], b[1 + (k + 2) % 3], 0, st);
+          launch_step(op, dtype, fl, lay, ncols, af, 0.5, 2.0, -1.0, b[0], b[1 + k % 3],
+                      b[1 + (k + 1) % 3], b[1 + (k + 2) % 3], 0, st);
         PF_HIP(hipEventRecord(e1, st));
         PF_HIP(hipEventSynchronize(e1));
         float ms = 0.f;

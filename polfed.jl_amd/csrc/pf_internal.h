@@ -128,6 +128,7 @@ enum : int {
   kOutCM = 8,   // write the output column-major (FP64) instead of interleaved
   kDot = 16,    // accumulate <P, out> per block (KPM)
   kStreamNT = 32,  // non-temporal hints on the streamed vectors
+  kRowPart = 64,   // spin chains: rows in the XCD partition order (drivers only)
 };
 
 struct Layout {
@@ -142,6 +143,7 @@ struct Layout {
   int wide;        // 1: twice the neighbour gathers in flight per batch (spin chains)
   int shfl;        // 1: Quantum-Sun grain block by cross-lane shuffles (else reloads)
   int occ;         // 1: spin-chain kernel held to 8 waves per SIMD (64 VGPRs)
+  int part;        // 1: spin-chain drivers run rows in the XCD partition order
 };
 
 // Launch configuration of the row-interleaved step kernel for one (dtype, s).
@@ -153,6 +155,7 @@ struct TuneCfg {
   int wide = -1;  // wide gather batches (-1 = default)
   int shfl = -1;  // Quantum-Sun grain shuffles (-1 = default)
   int occ = -1;   // spin-chain 8-waves-per-SIMD variant (-1 = default)
+  int part = -1;  // XCD partition row order in the drivers (-1 = default)
 };
 
 }  // namespace pf
@@ -179,6 +182,13 @@ struct pf_op {
   int qlb = 0;                 // Quantum Sun: low bits of the diagonal split
   void* rtab = nullptr;        // spin chain: u16 TA[2^m] then u32 TB[2^(L-m)] (rank tables)
   size_t rtab_bytes = 0;       // bytes of rtab (dynamic LDS of the step kernel)
+  // Spin chain with rank tables: the same basis in the XCD partition order
+  // (segments of equal high bits in reverse Cuthill-McKee order), used by the
+  // row-interleaved drivers; the hook entry points keep the reference order.
+  uint32_t* pstates = nullptr;  // basis states in partition order
+  double* pdiag = nullptr;      // diagonal in partition order
+  void* prtab = nullptr;        // TA, then TB of the partition order
+  uint32_t* pperm = nullptr;    // partition row -> reference rank
   pf::DevBuf fbuf;             // filter / KPM row-interleaved work blocks
   pf::TuneCfg force;           // pf_op_set_tuning overrides
   std::map<std::pair<int, int>, pf::TuneCfg> tuned;  // autotuned (dtype, s) configs
@@ -203,12 +213,13 @@ void launch_step(pf_op* op, int dtype, int flags, const Layout& lay, int ncols, 
                  const void* P = nullptr, int dot_blocks = 0);
 
 Layout interleaved_layout(int dtype, int s, int V = 0, int xcd = 1, int nt = 1, int pb = 0,
-                          int wide = 0, int shfl = 1, int occ = 0);
+                          int wide = 0, int shfl = 1, int occ = 0, int part = 0);
 Layout tuned_layout(pf_op* op, int dtype, int s);
 Layout colmajor_layout(int64_t ld);
 
-// Row-interleaved pack: dst[r*s + c] = (T)(src[r + c*ld] * colscale[c]) (colscale may be null).
+// Row-interleaved pack: dst[r*s + c] = (T)(src[perm[r] + c*ld] * colscale[c])
+// (colscale may be null; perm null is the identity).
 void pack_block(int dtype, const double* src, int64_t ld, void* dst, int64_t D, int s,
-                const double* colscale_dev, hipStream_t st);
+                const double* colscale_dev, hipStream_t st, const uint32_t* perm = nullptr);
 
 }  // namespace pf

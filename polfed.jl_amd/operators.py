@@ -133,7 +133,7 @@ class Operator:
     def set_tuning(self, xcd: int | None = None, vec: int | None = None, nt: int | None = None,
                    pblocks: int | None = None, wide: int | None = None,
                    grain_shuffle: int | None = None, occupancy: int | None = None,
-                   reset: bool = False):
+                   row_partition: int | None = None, reset: bool = False):
         """Force launch knobs of the row-interleaved kernels (see pf_op_set_tuning)."""
         if reset:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_RESET, 0))
@@ -146,6 +146,8 @@ class Operator:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_GRAIN_SHUFFLE, int(grain_shuffle)))
         if occupancy is not None:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_OCCUPANCY, int(occupancy)))
+        if row_partition is not None:
+            check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_ROW_PARTITION, int(row_partition)))
         if xcd is not None:
             check(self.lib.pf_op_set_tuning(self.op, _lib.PF_TUNE_XCD_REMAP, int(xcd)))
         if vec is not None:

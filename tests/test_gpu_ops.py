@@ -125,6 +125,43 @@ def test_filter_persistent_grid_settings_agree(dev, pblocks, wide, vec, occ):
     assert _rel(Y, oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)) < 1e-12
 
 
+PARTITION_CHAINS = CHAINS + [
+    ("xxz_L16_periodic", 16, 8, lambda L: om.xxz_bonds(L, 1.0, 0.6, "periodic"), 6),
+    ("j1j2_L15_open_Lup6", 15, 6, lambda L: om.j1j2_bonds(L, 1.0, 0.4, 1.0, 0.9, "open"), 7),
+]
+
+
+@pytest.mark.parametrize("name,L,Lup,bfn,seed", PARTITION_CHAINS)
+def test_row_partition_order_changes_nothing(dev, name, L, Lup, bfn, seed):
+    """The filter and KPM drivers run spin chains on the basis reordered by
+    segments (XCD partition); inputs and outputs stay in the reference order
+    and every row sums the same terms in the same order, so the filter output
+    is bit-identical with the partition on and off (FP64 and FP32), and the
+    KPM moments differ only by the order of the dot-product sums."""
+    from polfed_amd.blockops import fill_probes
+    op, H = _chain(dev, L, Lup, bfn, seed)
+    g = abs(H).sum(axis=1).max()  # Gershgorin bound of the spectrum
+    op.set_rescale(-g, g)
+    a, b = (op.Emax - op.Emin) / 2, (op.Emax + op.Emin) / 2
+    Ht = (H - b * sp.identity(H.shape[0])) / a
+    c = oc.filter_coefficients(0.15, 60)
+    X = np.random.default_rng(seed).standard_normal((H.shape[0], 8))
+    Xd = dev.to_device(X)
+    out = {}
+    for part in (1, 0):
+        op.set_tuning(row_partition=part)
+        out[part] = [dev.to_host(op.filter(c, Xd, dtype=dt)) for dt in ("f64", "f32")]
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert _rel(out[1][0], oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)) < 1e-12
+    P = fill_probes(dev, H.shape[0], 12, seed=seed)
+    mus = {}
+    for part in (1, 0):
+        op.set_tuning(row_partition=part)
+        mus[part] = op.kpm_moments(40, P)
+    assert np.max(np.abs(mus[1] - mus[0])) / H.shape[0] < 1e-13
+
+
 @pytest.mark.parametrize("L_grain,L_loc", [(1, 7), (2, 6), (4, 5)])
 @pytest.mark.parametrize("s", [1, 3, 8, 16])
 @pytest.mark.parametrize("shuffle", [1, 0])

@@ -150,19 +150,21 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& 
     const int64_t row = (lb * A.rpt + u) * A.rpb + rloc;
     if (row >= I.D) break;
     const int64_t o = row * A.s + cofs;
-    const vt xv = vload<T, V>(X + o);
-    const vt b3v = hasB3 ? vload<T, V>(B3 + o) : (vt)(T)0;
+    // (the rotating buffers are picked from a local table: global-space
+    // accesses keep them off the flat path)
+    const vt xv = *as_global(reinterpret_cast<const vt*>(X + o));
+    const vt b3v = hasB3 ? *as_global(reinterpret_cast<const vt*>(B3 + o)) : (vt)(T)0;
     vt hv = (vt)(T)0;
     if (hasH) hv = qsun_hrow_dg<T, V>(q, (uint32_t)row, sG, G_);
     vt out = cx * xv;
     if (hasH) out += ch * hv;
     if (hasB3) out += cb * b3v;
     if (k >= 1) {
-      vstore<T, V>(OUT + o, out);
+      *as_global(reinterpret_cast<vt*>(OUT + o)) = out;
     } else {
 #pragma unroll
       for (int v = 0; v < V; ++v)
-        dy.Y[row + (int64_t)(gl * V + v) * dy.ldy] = (double)vget<T, V>(out, v);
+        as_global(dy.Y)[row + (int64_t)(gl * V + v) * dy.ldy] = (double)vget<T, V>(out, v);
     }
   }
 }

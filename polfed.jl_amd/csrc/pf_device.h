@@ -45,6 +45,27 @@ template <typename T, int V>
 __device__ __forceinline__ void vstore_nt(T* p, typename VT<T, V>::t v) {
   __builtin_nontemporal_store(v, reinterpret_cast<typename VT<T, V>::t*>(p));
 }
+// The same pointer in the global address space (for pointers read from
+// device-side tables, which are otherwise generic and use flat instructions).
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* as_global(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
+// Vector load at byte offset off from base, known to be global memory (a
+// pointer read from a device-side table is otherwise generic and compiles to
+// flat loads with 64-bit address arithmetic).
+template <typename T, int V>
+__device__ __forceinline__ typename VT<T, V>::t gload(const char* base, uint32_t off) {
+  typedef typename VT<T, V>::t vt;
+  typedef const __attribute__((address_space(1))) vt* gp;
+  typedef const __attribute__((address_space(1))) char* gc;
+  return *(gp)((gc)base + off);
+}
 template <typename T, int V>
 __device__ __forceinline__ T vget(typename VT<T, V>::t v, int i) {
   if constexpr (V == 1) {

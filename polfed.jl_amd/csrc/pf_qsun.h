@@ -34,8 +34,10 @@ struct QsunRow {
   const void* sC;        // scaled outer couplings, type T (LDS)
 };
 
+// Byte offset of row r (P2: the row stride is 2^rsh bytes).
+template <bool P2>
 __device__ __forceinline__ uint32_t qsun_off(const QsunRow& q, uint32_t r) {
-  return (q.rsh >= 0 ? (r << q.rsh) : r * q.rsb) + q.cofs_b;
+  return (P2 ? (r << q.rsh) : r * q.rsb) + q.cofs_b;
 }
 
 // Stage the per-block LDS tables: sF (flip masks, shifted), sC (T couplings),
@@ -85,8 +87,11 @@ constexpr int qsun_flip_batch() {
 
 // H~ b2 at row r for this lane's V columns, DG = 2^Lg; sG holds the scaled
 // grain matrix (LDS).  Every load of a group (diagonal tables, grain block, a
-// batch of flips) is issued before the first of them is used.
-template <typename T, int V, int DG>
+// batch of flips) is issued before the first of them is used; a batch reads
+// its flip masks from LDS before one wait, and P2 (row stride a power of two,
+// masks pre-shifted) makes each flip offset one XOR and one add, with no
+// branch per flip.
+template <typename T, int V, int DG, bool P2>
 __device__ __forceinline__ typename VT<T, V>::t qsun_hrow(const QsunRow& q, uint32_t r,
                                                           const T* sG, int G) {
   typedef typename VT<T, V>::t vt;
@@ -94,20 +99,21 @@ __device__ __forceinline__ typename VT<T, V>::t qsun_hrow(const QsunRow& q, uint
   constexpr int GB = DG < NBF ? DG : NBF;  // grain loads per batch (load path)
   const T* sC = reinterpret_cast<const T*>(q.sC);
   const int a = (int)(r & (uint32_t)(DG - 1));
-  const double dl = q.dlo[r & ((1u << q.lb) - 1u)];
-  const double dh = q.dhi[r >> q.lb];
-  const vt xr = *reinterpret_cast<const vt*>(q.b2 + qsun_off(q, r));
-  const uint32_t rs_r = q.rsh >= 0 ? (r << q.rsh) : r;
+  const double dl = as_global(q.dlo)[r & ((1u << q.lb) - 1u)];
+  const double dh = as_global(q.dhi)[r >> q.lb];
+  const vt xr = gload<T, V>(q.b2, qsun_off<P2>(q, r));
+  const uint32_t rs_r = P2 ? (r << q.rsh) : r;
   vt acc = (vt)(T)0;
   // outer flips: whole batches of NBF (q.nloc is padded to a multiple of NBF)
   for (int j0 = 0; j0 < q.nloc; j0 += NBF) {
+    uint32_t f[NBF];
+#pragma unroll
+    for (int u = 0; u < NBF; ++u) f[u] = q.sF[j0 + u];
     vt x[NBF];
 #pragma unroll
     for (int u = 0; u < NBF; ++u) {
-      const int j = j0 + u;
-      const uint32_t o = q.rsh >= 0 ? ((rs_r ^ q.sF[j]) + q.cofs_b)
-                                    : ((r ^ q.sF[j]) * q.rsb + q.cofs_b);
-      x[u] = *reinterpret_cast<const vt*>(q.b2 + o);
+      const uint32_t o = P2 ? ((rs_r ^ f[u]) + q.cofs_b) : ((r ^ f[u]) * q.rsb + q.cofs_b);
+      x[u] = gload<T, V>(q.b2, o);
     }
 #pragma unroll
     for (int u = 0; u < NBF; ++u) acc += sC[j0 + u] * x[u];
@@ -119,13 +125,12 @@ __device__ __forceinline__ typename VT<T, V>::t qsun_hrow(const QsunRow& q, uint
 #pragma unroll
     for (int b = 1; b < DG; ++b) acc += sG[a * DG + (a ^ b)] * shfl_xor_vec<T, V>(xr, b * G);
   } else {
-    const uint32_t gbase = qsun_off(q, r & ~(uint32_t)(DG - 1));
+    const uint32_t gbase = qsun_off<P2>(q, r & ~(uint32_t)(DG - 1));
 #pragma unroll
     for (int b0 = 0; b0 < DG; b0 += GB) {
       vt xg[GB];
 #pragma unroll
-      for (int b = 0; b < GB; ++b)
-        xg[b] = *reinterpret_cast<const vt*>(q.b2 + gbase + (uint32_t)(b0 + b) * q.rsb);
+      for (int b = 0; b < GB; ++b) xg[b] = gload<T, V>(q.b2, gbase + (uint32_t)(b0 + b) * q.rsb);
 #pragma unroll
       for (int b = 0; b < GB; ++b) {
         const int bb = b0 + b;
@@ -138,18 +143,24 @@ __device__ __forceinline__ typename VT<T, V>::t qsun_hrow(const QsunRow& q, uint
   return acc;
 }
 
-// qsun_hrow for the runtime grain size q.dg (2, 4, 8 or 16), G lanes per row.
-// Every lane of a grain block must take part (rows per pass are whole
-// blocks and D is a multiple of 2^Lg).
+// qsun_hrow for the runtime grain size q.dg (2, 4, 8 or 16) and row-stride
+// shape, G lanes per row.  Every lane of a grain block must take part (rows
+// per pass are whole blocks and D is a multiple of 2^Lg).
+template <typename T, int V, bool P2>
+__device__ __forceinline__ typename VT<T, V>::t qsun_hrow_dg2(const QsunRow& q, uint32_t r,
+                                                              const T* sG, int G) {
+  switch (q.dg) {  // uniform
+    case 2: return qsun_hrow<T, V, 2, P2>(q, r, sG, G);
+    case 4: return qsun_hrow<T, V, 4, P2>(q, r, sG, G);
+    case 16: return qsun_hrow<T, V, 16, P2>(q, r, sG, G);
+    default: return qsun_hrow<T, V, 8, P2>(q, r, sG, G);
+  }
+}
 template <typename T, int V>
 __device__ __forceinline__ typename VT<T, V>::t qsun_hrow_dg(const QsunRow& q, uint32_t r,
                                                              const T* sG, int G) {
-  switch (q.dg) {  // uniform
-    case 2: return qsun_hrow<T, V, 2>(q, r, sG, G);
-    case 4: return qsun_hrow<T, V, 4>(q, r, sG, G);
-    case 16: return qsun_hrow<T, V, 16>(q, r, sG, G);
-    default: return qsun_hrow<T, V, 8>(q, r, sG, G);
-  }
+  return q.rsh >= 0 ? qsun_hrow_dg2<T, V, true>(q, r, sG, G)
+                    : qsun_hrow_dg2<T, V, false>(q, r, sG, G);
 }
 
 }  // namespace pf

@@ -169,27 +169,51 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& 
   }
 }
 
-// One Clenshaw step of every active instance whose arithmetic is T.
+// This block's instance (launch position lo), its block index lb inside the
+// instance and the step k; false when the instance is idle at this step.
 // kexpl >= 0: the step index; kexpl < 0: k = kstate[0] - j (graph nodes).
-template <typename T, int V>
-__global__ __launch_bounds__(256) void qsun_batch_step_kernel(const BatchArgs A, int kexpl,
-                                                              int j) {
+__device__ __forceinline__ bool batch_locate(const BatchArgs& A, int kexpl, int j, int& lo,
+                                             BatchDyn& dy, int& k, int64_t& lb) {
   const int64_t nb = gridDim.x;
   int64_t b = blockIdx.x;
   if (A.xcd) b = xcd_block(b, nb);
-  int lo = 0, hi = A.n;  // blk0[lo] <= b < blk0[hi]
+  lo = 0;
+  int hi = A.n;  // blk0[lo] <= b < blk0[hi]
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
     if (A.blk0[mid] <= b) lo = mid;
     else hi = mid;
   }
-  const BatchDyn dy = A.dyn[lo];
-  const int k = kexpl >= 0 ? kexpl : A.kstate[0] - j;
+  dy = A.dyn[lo];
+  k = kexpl >= 0 ? kexpl : A.kstate[0] - j;
+  lb = b - A.blk0[lo];
+  return !(k < 0 || dy.K < 0 || k > dy.K);  // uniform over the block
+}
+
+// One Clenshaw step of every active instance whose arithmetic is T.
+template <typename T, int V>
+__global__ __launch_bounds__(256) void qsun_batch_step_kernel(const BatchArgs A, int kexpl,
+                                                              int j) {
+  int lo, k;
+  BatchDyn dy;
+  int64_t lb;
   constexpr int kDt = sizeof(T) == 8 ? PF_F64 : PF_F32;
-  if (k < 0 || dy.K < 0 || k > dy.K || dy.dtype != kDt) return;  // uniform over the block
-  const BatchInst& I = A.insts[lo];
-  const int64_t lb = b - A.blk0[lo];
-  batch_body<T, V>(A, I, dy, k, lb);
+  if (!batch_locate(A, kexpl, j, lo, dy, k, lb) || dy.dtype != kDt) return;
+  batch_body<T, V>(A, A.insts[lo], dy, k, lb);
+}
+
+// One Clenshaw step of every active instance, FP64 and FP32 instances in the
+// same launch (each block takes its instance's arithmetic): an application
+// with both kinds costs one launch per step, not two over the whole grid.
+template <int V>
+__global__ __launch_bounds__(256) void qsun_batch_step_kernel_mixed(const BatchArgs A, int kexpl,
+                                                                    int j) {
+  int lo, k;
+  BatchDyn dy;
+  int64_t lb;
+  if (!batch_locate(A, kexpl, j, lo, dy, k, lb)) return;
+  if (dy.dtype == PF_F64) batch_body<double, V>(A, A.insts[lo], dy, k, lb);
+  else batch_body<float, V>(A, A.insts[lo], dy, k, lb);
 }
 
 // dtype bit set of an application: bit 0 FP64 instances, bit 1 FP32 ones
@@ -341,11 +365,25 @@ struct pf_batch {
     }
 #undef PF_BATCH_LAUNCH
   }
-  // one step launch per arithmetic present in the application (mask)
+  // one step launch covering the arithmetics present in the application (mask)
   void launch(const BatchArgs& A, int64_t grid, int kexpl, int j, hipStream_t st, int mask) {
     if (grid <= 0) return;
-    if (mask & kMaskF64) launch_t<double>(A, grid, kexpl, j, st);
-    if (mask & kMaskF32) launch_t<float>(A, grid, kexpl, j, st);
+    if (mask == (kMaskF64 | kMaskF32)) {
+#define PF_BATCH_LAUNCH(VV)                                                             \
+  hipLaunchKernelGGL((qsun_batch_step_kernel_mixed<VV>), dim3((unsigned)grid), dim3(256), \
+                     0, st, A, kexpl, j)
+      switch (V) {
+        case 2: PF_BATCH_LAUNCH(2); break;
+        case 4: PF_BATCH_LAUNCH(4); break;
+        case 8: PF_BATCH_LAUNCH(8); break;
+        default: PF_BATCH_LAUNCH(1); break;
+      }
+#undef PF_BATCH_LAUNCH
+    } else if (mask & kMaskF64) {
+      launch_t<double>(A, grid, kexpl, j, st);
+    } else if (mask & kMaskF32) {
+      launch_t<float>(A, grid, kexpl, j, st);
+    }
     PF_HIP(hipGetLastError());
   }
   // timing: capture with constant coefficients (no coefficient reads), for

@@ -88,6 +88,20 @@ int pf_op_create_qsun(pf_handle* h, int32_t L_grain, int32_t L_loc, double g0,
                       const double* H_grain, const int32_t* neighbors,
                       const double* couplings, const double* fields, pf_op** out);
 
+/* Packed sparse operator: any real symmetric matrix in the reference's
+ * value-bucket form, the output of get_diags_and_offdiagonals_by_value
+ * (src/Optimization/optimization.jl:134-182) that optimize_spectral_transform
+ * (:75-119) builds for polfed(mat, ...) and that optimized_mapping! applies
+ * (src/Optimization/cpu/single_row_mappings.jl:66-79): diag[D]; nvals distinct
+ * off-diagonal values vals[nvals] (ascending); for value v and row r the
+ * columns flat[starts[v*(D+1)+r] .. starts[v*(D+1)+r+1]) (0-based columns,
+ * absolute offsets into the concatenated flat array, one (D+1)-entry start
+ * table per value; the reference keeps one 1-based flat/starts pair per
+ * value).  Host arrays.  nvals <= 256; more than one value needs D <= 2^24. */
+int pf_op_create_packed(pf_handle* h, int64_t D, const double* diag, int32_t nvals,
+                        const double* vals, const int64_t* starts, const int32_t* flat,
+                        pf_op** out);
+
 int pf_op_destroy(pf_op* op);
 
 /* Hilbert-space dimension D of the operator (0 on error). */

@@ -12,9 +12,13 @@ import scipy.sparse as sp
 
 
 def round_key(x: float, digits: int = 15) -> float:
-    """``round_key`` (src/Optimization/optimization.jl:24): round to ``digits``
-    decimal digits (Python's correctly rounded ``round`` on binary doubles)."""
-    return round(float(x), digits)
+    """``round_key`` (src/Optimization/optimization.jl:24): Julia's
+    ``round(x; digits)`` is ``round(x * 10^digits) / 10^digits`` with ties to
+    even, in FP64 (base/floatfuncs.jl ``_round_invstep``); not Python's
+    correctly rounded decimal ``round``, which differs in the last bit for
+    some values."""
+    sc = 10.0 ** digits
+    return float(np.rint(float(x) * sc) / sc)
 
 
 def get_diags_and_offdiagonals_by_value(mat, tol=1e-13, round_digits=15):

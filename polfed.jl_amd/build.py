@@ -12,7 +12,7 @@ import subprocess
 PKG = pathlib.Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT = PKG / "lib" / "libpolfed_hip.so"
-SOURCES = ["ops.hip", "dense.hip", "batch.hip", "capi.cpp"]
+SOURCES = ["ops.hip", "dense.hip", "batch.hip", "packed.hip", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -15,6 +15,7 @@ replace the corresponding pieces with user callables, as in the reference
 """
 from __future__ import annotations
 
+import dataclasses
 import time
 from dataclasses import dataclass, field
 from typing import Optional
@@ -28,7 +29,7 @@ from .configs import (DoSConfig, FactorizationConfig, MappingConfig, PolfedDefau
                       TransformConfig)
 from .device import Device, default_device
 from .lanczos import LanczosResult, lanczos
-from .operators import Operator
+from .operators import Operator, PackedOperator
 
 
 @dataclass
@@ -103,16 +104,38 @@ def prepare_initial_state(dev: Device, x0) -> torch.Tensor:
     return q
 
 
-def _extremal(dev: Device, op: Operator, v0: torch.Tensor, which: str) -> float:
+def _extremal(dev: Device, op: Operator, v0: torch.Tensor, which: str, tol: float = 1e-14,
+              eigentol: float = 1e-8) -> float:
     """``lanczos(f!, v0, 1; which, maxdim=1000)`` probe of build_mapping_plan
-    (Plan.jl:136-168) with the unscaled operator."""
+    (Plan.jl:136-168) with the unscaled operator (tolerances of
+    optimize_spectral_transform's probe: optimization.jl:40-52)."""
     f = lambda X, Y: op.apply(X, Y)
-    res = lanczos(dev, f, v0, 1, maxdim=1000, which=which, tol=1e-14, eigentol=1e-8,
+    res = lanczos(dev, f, v0, 1, maxdim=1000, which=which, tol=tol, eigentol=eigentol,
                   mapvals=op, stage_timers=False)
     if res.vals.size == 0:
         raise RuntimeError(f"Failed to estimate {which} extremal eigenvalue with Lanczos. "
                            "Set MappingConfig(Emin=..., Emax=...) explicitly.")
     return float(res.vals[0] if which == "SR" else res.vals[-1])
+
+
+def _matrix_operator(H, mapping: MappingConfig, dev: Device):
+    """``polfed(mat, ...)`` (PolfedCore.jl:153-184): the matrix becomes the
+    packed value-bucket operator of optimize_spectral_transform
+    (optimization.jl:75-119); with ``mapping.optimize_mapping`` its extremal
+    eigenvalues come from that function's probes -- a random normalized start
+    vector, tol = eigentol = sqrt(eps) (optimization.jl:40-52,90-97) -- unless
+    Emin/Emax are given.  The packed operator serves both settings of
+    ``optimize_mapping`` (the reference's other branch multiplies by the
+    matrix itself, ``mul!``, the same operator)."""
+    op = PackedOperator.from_matrix(H, device=dev)
+    if mapping.optimize_mapping and (mapping.Emin is None or mapping.Emax is None):
+        v = np.random.default_rng(0).standard_normal(op.D)
+        v0 = dev.to_device(v / np.linalg.norm(v))
+        tol = float(np.sqrt(np.finfo(np.float64).eps))
+        Emin = mapping.Emin if mapping.Emin is not None else _extremal(dev, op, v0, "SR", tol, tol)
+        Emax = mapping.Emax if mapping.Emax is not None else _extremal(dev, op, v0, "LR", tol, tol)
+        mapping = dataclasses.replace(mapping, Emin=Emin, Emax=Emax)
+    return op, mapping
 
 
 def _filter_callable(op: Operator, mapping: MappingConfig, coeffs: np.ndarray, polytype: str,
@@ -232,7 +255,9 @@ def polfed(op: Operator, x0, howmany: int, target, *, produce_report: bool = Fal
            probes: Optional[torch.Tensor] = None, device: Optional[Device] = None):
     """Eigenpairs of ``op`` closest (by the polynomial filter) to ``target``.
 
-    Returns ``(vals, vecs)`` -- vals ascending (host numpy), vecs an
+    ``op`` is a GPU operator, or a real symmetric matrix (scipy sparse or
+    dense), which is packed into a :class:`PackedOperator` as the reference's
+    matrix method does.  Returns ``(vals, vecs)`` -- vals ascending (host numpy), vecs an
     (nconv, D) device block -- or ``(vals, vecs, report)`` with
     ``produce_report=True``.  ``probes`` (D x R device block) overrides the
     generated KPM probe vectors.
@@ -241,6 +266,9 @@ def polfed(op: Operator, x0, howmany: int, target, *, produce_report: bool = Fal
     transform = transform or TransformConfig()
     fact = fact or FactorizationConfig()
     dos = dos or DoSConfig()
+    if not isinstance(op, Operator):  # matrix input (scipy sparse or dense)
+        dev = device or default_device()
+        op, mapping = _matrix_operator(op, mapping, dev)
     dev = device or op.device or default_device()
     t_all = time.perf_counter()
     prep = _prepare(op, x0, howmany, target, mapping, transform, fact, dos, probes, dev)

@@ -21,55 +21,9 @@
 #include "pf_device.h"
 #include "pf_internal.h"
 #include "pf_qsun.h"
+#include "pf_step.h"
 
 namespace pf {
-
-// Streamed inputs of one row (X and b3), loaded before the neighbour gathers
-// so that their latency overlaps them.
-template <typename T, int V>
-__device__ __forceinline__ void step_load(int flags, int64_t off, const T* __restrict__ X,
-                                          const T* __restrict__ B3, typename VT<T, V>::t& xv,
-                                          typename VT<T, V>::t& b3v) {
-  typedef typename VT<T, V>::t vt;
-  const bool nt = flags & kStreamNT;
-  xv = (vt)(T)0;
-  b3v = (vt)(T)0;
-  if (flags & kHasX) xv = nt ? vload_nt<T, V>(X + off) : vload<T, V>(X + off);
-  if (flags & kHasB3) b3v = nt ? vload_nt<T, V>(B3 + off) : vload<T, V>(B3 + off);
-}
-
-// out = cx*X + ch*h + cb*b3, store (interleaved T, or FP64 column-major with
-// kOutCM at row operm[row], the reference rank, when the rows run in the
-// partition order), optional KPM dot <P, out>.
-template <typename T, int V>
-__device__ __forceinline__ void step_store(int flags, int64_t row, int64_t off, int out_col0,
-                                           T cx, T ch, T cb, typename VT<T, V>::t h,
-                                           typename VT<T, V>::t xv, typename VT<T, V>::t b3v,
-                                           void* __restrict__ OUT, int64_t out_ld,
-                                           const uint32_t* __restrict__ operm,
-                                           const T* __restrict__ Pr, double& dacc) {
-  typedef typename VT<T, V>::t vt;
-  vt o = (vt)(T)0;
-  const bool nt = flags & kStreamNT;
-  if (flags & kHasX) o = cx * xv;
-  if (flags & kHasH) o += ch * h;
-  if (flags & kHasB3) o += cb * b3v;
-  if (flags & kOutCM) {
-    double* Y = reinterpret_cast<double*>(OUT);
-    const int64_t yr = operm ? (int64_t)operm[row] : row;
-#pragma unroll
-    for (int v = 0; v < V; ++v) Y[yr + (int64_t)(out_col0 + v) * out_ld] = (double)vget<T, V>(o, v);
-  } else if (nt) {
-    vstore_nt<T, V>(reinterpret_cast<T*>(OUT) + off, o);
-  } else {
-    vstore<T, V>(reinterpret_cast<T*>(OUT) + off, o);
-  }
-  if (flags & kDot) {
-    const vt p = vload<T, V>(Pr + off);
-#pragma unroll
-    for (int v = 0; v < V; ++v) dacc += (double)vget<T, V>(p, v) * (double)vget<T, V>(o, v);
-  }
-}
 
 // ------------------------------------------------------ spin-chain kernel
 // Broadcast a 32-bit value from lane Q of this lane's group of G lanes (the
@@ -449,7 +403,7 @@ Layout colmajor_layout(int64_t ld) {
 
 // Blocks of one step launch: one per row group, capped by the persistent grid
 // (pblocks per CU) and, for KPM dot launches, by the number of partial slots.
-static int64_t step_grid_x(const pf_op* op, const Layout& lay, int flags, int dot_blocks) {
+int64_t step_grid_x(const pf_op* op, const Layout& lay, int flags, int dot_blocks) {
   const int64_t ngroups = (op->D + lay.rows_per_block - 1) / lay.rows_per_block;
   int64_t gx = ngroups;
   if (lay.pblocks > 0)
@@ -523,6 +477,11 @@ void launch_step(pf_op* op, int dtype, int flags, const Layout& lay, int ncols, 
                  void* out, int64_t out_ld, hipStream_t st, double* dot_part, const void* P,
                  int dot_blocks) {
   const int ny = lay.cstride ? ncols : 1;
+  if (op->kind == kPacked) {
+    launch_packed(op, dtype, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part,
+                  P, dot_blocks);
+    return;
+  }
   if (dtype == PF_F64) {
     switch (lay.V) {
       case 1: launch_typed<double, 1>(op, flags, lay, ny, af, cx, ch, cb, X, B2, B3, out, out_ld, st, dot_part, P, dot_blocks); break;
@@ -1113,6 +1072,9 @@ extern "C" int pf_op_destroy(pf_op* op) {
   if (op->prtab) (void)hipFree(op->prtab);
   if (op->pperm) (void)hipFree(op->pperm);
   if (op->qdiag) (void)hipFree(op->qdiag);
+  if (op->prow) (void)hipFree(op->prow);
+  if (op->pent) (void)hipFree(op->pent);
+  if (op->pvals) (void)hipFree(op->pvals);
   op->fbuf.release();
   op->kpart.release();
   delete op;
@@ -1171,6 +1133,8 @@ extern "C" int pf_op_connectivity(pf_op* op, int64_t row0, int64_t nrows, int32_
       (void)hipFree(dbi);
       (void)hipFree(dbj);
       (void)hipFree(dh);
+    } else if (op->kind == kPacked) {
+      packed_connectivity(op, row0, nrows, max_per_row, cols, vals, counts, diag, st);
     } else {
       hipLaunchKernelGGL(qsun_conn_kernel, dim3(nb), dim3(256), 0, st, op->qsun, row0, nrows,
                          max_per_row, cols, vals, counts, diag);

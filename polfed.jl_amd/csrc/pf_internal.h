@@ -112,7 +112,11 @@ struct QsunParams {
   double G[(1 << kMaxGrain) * (1 << kMaxGrain)];  // grain matrix, G[a*dg + b]
 };
 
-enum OpKind { kChain = 0, kQsun = 1 };
+enum OpKind { kChain = 0, kQsun = 1, kPacked = 2 };
+
+// Packed operator: at most this many distinct off-diagonal values when there
+// is more than one (the value index sits in the top 8 bits of an entry).
+constexpr int kMaxPackedVals = 256;
 
 // Affine map of the operator: H' = (H - shift) * scale (scale = 1/a, shift = b
 // for the rescaled operator; 1, 0 for the plain one).
@@ -179,6 +183,14 @@ struct pf_op {
   uint32_t* binom = nullptr;   // spin chain: C(p, t), p < 32, t < kRankT
   double* diag = nullptr;      // spin chain: unscaled diagonal, reference summation order
   double* qdiag = nullptr;     // Quantum Sun: Dlo[2^qlb] then Dhi[2^(Lg+Lloc-qlb)] (diagonal)
+  // Packed operator (diag in `diag`): row r's off-diagonal entries are
+  // pent[prow[r] .. prow[r+1]), ordered by value index then column; an entry is
+  // the column, with the value index in bits 24..31 when pnvals > 1.
+  int64_t* prow = nullptr;
+  uint32_t* pent = nullptr;
+  double* pvals = nullptr;     // the distinct off-diagonal values, ascending (unscaled)
+  int pnvals = 0;
+  int64_t pnnz = 0;
   int qlb = 0;                 // Quantum Sun: low bits of the diagonal split
   void* rtab = nullptr;        // spin chain: u16 TA[2^m] then u32 TB[2^(L-m)] (rank tables)
   size_t rtab_bytes = 0;       // bytes of rtab (dynamic LDS of the step kernel)
@@ -216,6 +228,21 @@ Layout interleaved_layout(int dtype, int s, int V = 0, int xcd = 1, int nt = 1, 
                           int wide = 0, int shfl = 1, int occ = 0, int part = 0);
 Layout tuned_layout(pf_op* op, int dtype, int s);
 Layout colmajor_layout(int64_t ld);
+
+// Blocks of one step launch of the matrix-free or packed kernels (ops.hip):
+// one per row group, capped by the persistent grid and the KPM partial slots.
+int64_t step_grid_x(const pf_op* op, const Layout& lay, int flags, int dot_blocks);
+
+// One fused step of a packed operator (packed.hip); arguments as launch_step.
+void launch_packed(pf_op* op, int dtype, int flags, const Layout& lay, int ncols_grid, Affine af,
+                   double cx, double ch, double cb, const void* X, const void* B2, const void* B3,
+                   void* out, int64_t out_ld, hipStream_t st, double* dot_part, const void* P,
+                   int dot_blocks);
+
+// Connectivity rows of a packed operator (packed.hip).
+void packed_connectivity(pf_op* op, int64_t row0, int64_t nrows, int32_t max_per_row,
+                         int64_t* cols, double* vals, int32_t* counts, double* diag,
+                         hipStream_t st);
 
 // Row-interleaved pack: dst[r*s + c] = (T)(src[perm[r] + c*ld] * colscale[c])
 // (colscale may be null; perm null is the identity).

@@ -238,3 +238,39 @@ class QSunOperator(Operator):
         super().__init__(device, op)
         self.L_loc, self.L_grain, self.g0 = L_loc, L_grain, g0
         self.disorder, self.neighbors, self.couplings, self.H_grain = hz, nb, cp, G
+
+
+class PackedOperator(Operator):
+    """Any real symmetric matrix in the reference's value-bucket form
+    (``optimize_spectral_transform``, src/Optimization/optimization.jl:75-119,
+    and its packed row kernel, src/Optimization/cpu/single_row_mappings.jl:66-79)
+    -- the operator ``polfed(mat, ...)`` runs on.  ``diag``, ``vals``,
+    ``starts`` and ``flat`` as returned by :func:`packing.pack_by_value`."""
+
+    def __init__(self, diag, vals, starts, flat, device: Device | None = None):
+        device = device or default_device()
+        dg = np.ascontiguousarray(diag, dtype=np.float64)
+        D = dg.size
+        vv = np.ascontiguousarray(vals, dtype=np.float64).reshape(-1)
+        st = np.ascontiguousarray(starts, dtype=np.int64)
+        fl = np.ascontiguousarray(flat, dtype=np.int32).reshape(-1)
+        if st.shape != (vv.size, D + 1):
+            raise ValueError(f"starts must have shape ({vv.size}, {D + 1}), got {st.shape}")
+        if vv.size and st[-1, -1] > fl.size:
+            raise ValueError("starts point past the end of flat")
+        op = ctypes.c_void_p()
+        i64p = st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        with torch.cuda.device(device.index):
+            check(device.lib.pf_op_create_packed(device.h, D, dptr(dg), vv.size, dptr(vv), i64p,
+                                                 iptr(fl), ctypes.byref(op)))
+        super().__init__(device, op)
+        self.diag, self.vals, self.starts, self.flat = dg, vv, st, fl
+
+    @classmethod
+    def from_matrix(cls, H, device: Device | None = None, tol: float = 1e-13,
+                    round_digits: int = 15) -> "PackedOperator":
+        """Pack a scipy sparse or dense matrix (``get_diags_and_offdiagonals_by_value``,
+        src/Optimization/optimization.jl:134-182) and build the operator."""
+        from .packing import pack_by_value
+        return cls(*pack_by_value(H, tol, round_digits), device=device)
+

@@ -1,0 +1,166 @@
+"""GPU parity of the packed sparse operator (pf_op_create_packed: any real
+symmetric matrix in the reference's value-bucket form) and of polfed(matrix),
+against scipy, the oracle and dense eigensolvers."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from oracle import clenshaw as oc
+from oracle import dos as odos
+from oracle import models as om
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _random_symmetric(D, nvals, density, seed, empty_row=None):
+    """Sparse symmetric matrix with nvals distinct off-diagonal values."""
+    rng = np.random.default_rng(seed)
+    A = sp.random(D, D, density=density, random_state=seed, format="coo")
+    levels = rng.uniform(-1, 1, nvals)
+    A = sp.coo_matrix((rng.choice(levels, A.nnz), (A.row, A.col)), shape=(D, D))
+    A = sp.triu(A, 1)
+    A = (A + A.T).tolil()
+    if empty_row is not None:
+        A[empty_row, :] = 0
+        A[:, empty_row] = 0
+    A.setdiag(rng.uniform(-2, 2, D))
+    return A.tocsr()
+
+
+def _packed_matrix(diag, vals, starts, flat):
+    """The matrix the packed arrays describe (rounded values), from the arrays."""
+    D = len(diag)
+    rows, cols, vv = [], [], []
+    for v in range(len(vals)):
+        for r in range(D):
+            cs = flat[starts[v, r]:starts[v, r + 1]]
+            rows += [r] * len(cs)
+            cols += list(cs)
+            vv += [vals[v]] * len(cs)
+    return (sp.coo_matrix((vv, (rows, cols)), shape=(D, D)) + sp.diags(diag)).tocsr()
+
+
+def _matrices():
+    bonds, Js, Ds = om.xxz_bonds(12, 1.0, 1.0, "periodic")
+    h = np.random.default_rng(1).uniform(-2, 2, 12)
+    xxz = om.spin_half_hamiltonian(12, bonds, Js, Ds, h, 6)
+    bonds, Js, Ds = om.j1j2_bonds(12, 1.0, 0.5, 1.0, 0.7, "periodic")
+    j1j2 = om.spin_half_hamiltonian(12, bonds, Js, Ds, h, 6)
+    draws = om.qsun_draws(6, 3, 0.75, rng=np.random.default_rng(3))
+    qsun = om.qsun_hamiltonian_from_draws(6, 3, 1.0, *draws)
+    return {"xxz_L12": xxz, "j1j2_L12": j1j2, "qsun_6_3": qsun,
+            "random_200vals": _random_symmetric(1500, 200, 0.01, 4, empty_row=7)}
+
+
+MATS = _matrices()
+
+
+@pytest.mark.parametrize("name", list(MATS))
+def test_packed_apply_filter_kpm(dev, name):
+    from polfed_amd.blockops import fill_probes
+    from polfed_amd.operators import PackedOperator
+    from polfed_amd.packing import pack_by_value
+    packed = pack_by_value(MATS[name])
+    H = _packed_matrix(*packed)
+    op = PackedOperator(*packed, device=dev)
+    D = H.shape[0]
+    assert op.D == D
+    # connectivity: exactly the packed matrix
+    cols, vals, counts, diag = op.connectivity(max_per_row=int(np.diff(H.indptr).max()) + 1)
+    assert np.array_equal(diag, H.diagonal())
+    rr = np.repeat(np.arange(D), counts)
+    cc = np.concatenate([cols[r, :counts[r]] for r in range(D)])
+    vv = np.concatenate([vals[r, :counts[r]] for r in range(D)])
+    assert abs(sp.coo_matrix((vv, (rr, cc)), shape=(D, D)) + sp.diags(diag) - H).max() == 0.0
+    for s in (1, 3, 8):
+        X = np.random.default_rng(s).standard_normal((D, s))
+        assert _rel(dev.to_host(op.apply(dev.to_device(X))), H @ X) < 1e-13, s
+        Y32 = dev.to_host(op.apply(dev.to_device(X).float(), dtype="f32"))
+        assert _rel(Y32, H @ X) < 1e-5, s
+    g = abs(H).sum(axis=1).max()
+    op.set_rescale(-g, g)
+    Ht = H / g
+    c = oc.filter_coefficients(0.2, 90)
+    X = np.random.default_rng(9).standard_normal((D, 8))
+    assert _rel(dev.to_host(op.filter(c, dev.to_device(X))), oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)) < 1e-12
+    P = fill_probes(dev, D, 10, seed=2)
+    mu = op.kpm_moments(40, P)
+    mu_ref = odos.dos_moments(lambda v: Ht @ v, 40, dev.to_host(P))
+    assert np.max(np.abs(mu - mu_ref)) / D < 1e-12
+
+
+def test_packed_clenshaw_hooks_closed_form(dev):
+    """recurrence b1 = c X + 2 H~ b2 - b3 and final sum Y = c0 X + H~ b1 - b2 on
+    the caller's column-major blocks (test/optimization.jl:81-99)."""
+    from polfed_amd.operators import PackedOperator
+    from polfed_amd.packing import pack_by_value
+    packed = pack_by_value(MATS["random_200vals"])
+    H = _packed_matrix(*packed)
+    op = PackedOperator(*packed, device=dev)
+    op.set_rescale(-3.0, 5.0)
+    Ht = (H - 1.0 * sp.identity(H.shape[0])) / 4.0
+    rng = np.random.default_rng(11)
+    X, b2, b3 = (rng.standard_normal((H.shape[0], 4)) for _ in range(3))
+    b1 = dev.to_device(np.zeros_like(X))
+    op.recurrence(b1, dev.to_device(b2), dev.to_device(b3), 0.3, dev.to_device(X))
+    assert _rel(dev.to_host(b1), 0.3 * X + 2 * Ht @ b2 - b3) < 1e-13
+    Y = dev.to_device(np.zeros_like(X))
+    op.finalsum(dev.to_device(b2), dev.to_device(b3), 0.7, Y, dev.to_device(X))
+    assert _rel(dev.to_host(Y), 0.7 * X + Ht @ b2 - b3) < 1e-13
+
+
+def test_packed_equals_matrix_free_chain(dev):
+    """Two independent GPU operators for one XXZ chain (the packed matrix and
+    the matrix-free rank-table kernel) give the same filter output."""
+    from polfed_amd.operators import PackedOperator, SpinChainOperator
+    L, Lup = 14, 7
+    bonds, Js, Ds = om.xxz_bonds(L, 1.0, 0.8, "periodic")
+    h = np.random.default_rng(5).uniform(-1, 1, L)
+    H = om.spin_half_hamiltonian(L, bonds, Js, Ds, h, Lup)
+    pk = PackedOperator.from_matrix(H, device=dev)
+    mf = SpinChainOperator(L, Lup, bonds, Js, Ds, h, device=dev)
+    for op in (pk, mf):
+        op.set_rescale(-12.0, 12.0)
+    c = oc.filter_coefficients(-0.3, 120)
+    X = dev.to_device(np.random.default_rng(6).standard_normal((H.shape[0], 8)))
+    assert _rel(dev.to_host(pk.filter(c, X)), dev.to_host(mf.filter(c, X))) < 1e-12
+
+
+def test_polfed_on_a_matrix(dev):
+    """polfed(mat, ...) (PolfedCore.jl:153-184): the matrix is packed, the
+    bounds come from the sqrt(eps) probes of optimize_spectral_transform, and
+    the returned pairs are eigenpairs of the matrix to 1e-10."""
+    from polfed_amd import FactorizationConfig, polfed
+    H = MATS["random_200vals"]
+    w = np.linalg.eigvalsh(H.toarray())
+    x0 = np.random.default_rng(1).standard_normal(H.shape[0])
+    vals, vecs, rep = polfed(H, x0, 10, 0.1, produce_report=True,
+                             fact=FactorizationConfig(eigentol=1e-10))
+    width = w[-1] - w[0]
+    assert len(vals) == 10
+    assert rep.Emin >= w[0] - 1e-6 * width and rep.Emax <= w[-1] + 1e-6 * width
+    assert max(np.min(np.abs(w - v)) for v in vals) <= 1e-10 * width
+    psi = dev.to_host(vecs)
+    from polfed_amd.packing import pack_by_value
+    Hp = _packed_matrix(*pack_by_value(H))
+    assert np.linalg.norm(Hp @ psi - psi * vals, axis=0).max() <= 1e-10
+    # the 10 returned values are the 10 eigenvalues nearest the target's
+    # position in the rescaled spectrum (a contiguous run of the spectrum)
+    idx = np.searchsorted(w, vals - 1e-9 * width)
+    assert np.all(np.diff(idx) == 1)
+
+
+def test_packed_rejects_too_many_values(dev):
+    from polfed_amd import PolfedHIPError
+    from polfed_amd.operators import PackedOperator
+    D = 40
+    rows, cols = np.triu_indices(D, 1)
+    v = np.arange(1, rows.size + 1) * 1e-3  # 780 distinct values
+    H = sp.coo_matrix((np.r_[v, v], (np.r_[rows, cols], np.r_[cols, rows])), shape=(D, D))
+    with pytest.raises(PolfedHIPError):
+        PackedOperator.from_matrix(H, device=dev)

@@ -121,3 +121,58 @@ def test_defaults_and_configs():
     assert isinstance(fc.rot, polfed_amd.FullRO) and fc.rot.passes == 2
     d = polfed_amd.DoSConfig()
     assert (d.N, d.R, d.kernel) == (250, 300, "Jackson")
+
+
+# ------------------------------------------------------------ packed operator
+def _per_value_1based(vals, starts, flat):
+    """The C ABI layout -> the reference's per-value 1-based (value, flat, starts)."""
+    D = starts.shape[1] - 1
+    out = []
+    for v in range(len(vals)):
+        f = flat[starts[v, 0]:starts[v, D]] + 1
+        out.append((vals[v], list(f), list(starts[v, :D] - starts[v, 0] + 1)))
+    return out
+
+
+def test_pack_by_value_reference_known_answer():
+    """The product's packing (host side of pf_op_create_packed) reproduces the
+    flat/starts arrays of test/optimization.jl:41-55, sparse and dense input."""
+    import scipy.sparse as sps
+    from polfed_amd.packing import pack_by_value
+    k = KAT["packing_5x5"]
+    H = sps.lil_matrix((5, 5))
+    for i, v in enumerate(k["diag"]):
+        H[i, i] = v
+    for i, j, v in k["offdiag_entries"]:
+        H[i - 1, j - 1] = H[j - 1, i - 1] = v
+    for M in (H.tocsc(), H.toarray()):
+        diag, vals, starts, flat = pack_by_value(M)
+        assert list(diag) == k["diag"]
+        got = _per_value_1based(vals, starts, flat)
+        assert len(got) == len(k["expected_buckets_1based"])
+        for (v, f, s), e in zip(got, k["expected_buckets_1based"]):
+            assert v == e["value"] and f == e["flat"] and s == e["starts"]
+
+
+def test_pack_by_value_matches_oracle_on_random_matrices():
+    """Same buckets as the oracle's restatement on matrices with rounding-level
+    value duplicates, tiny entries below tol and empty rows."""
+    import scipy.sparse as sps
+    from oracle import packing as opk
+    from polfed_amd.packing import pack_by_value
+    rng = np.random.default_rng(5)
+    D = 60
+    A = sps.random(D, D, density=0.1, random_state=7, format="coo")
+    vals = rng.choice([-1.0, 0.5, 0.5 + 1e-17, 2.25, 1e-14], size=A.nnz)
+    A = sps.coo_matrix((vals, (A.row, A.col)), shape=(D, D)).tocsr()
+    A = (A + A.T).tolil()
+    A[3, :] = 0
+    A[:, 3] = 0
+    A.setdiag(rng.standard_normal(D))
+    A = A.tocsc()
+    diag, vv, starts, flat = pack_by_value(A)
+    d_ref, buckets = opk.get_diags_and_offdiagonals_by_value(A)
+    assert np.array_equal(diag, d_ref)
+    assert len(vv) == len(buckets)
+    for (v, f, s), (rv, rf, rs) in zip(_per_value_1based(vv, starts, flat), buckets):
+        assert v == rv and f == list(rf + 1) and s == list(rs + 1)

@@ -1,7 +1,11 @@
-"""Model constructors mirroring ``Polfed.Models`` (src/Models/Models.jl:85-117).
+"""Model constructors mirroring ``Polfed.Models`` (src/Models/Models.jl:85-117)
+and ``Polfed.QSun`` (src/QSun).
 
-They return matrix-free device operators instead of SparseMatrixCSC: the
-basis, connectivity and matrix elements are computed by the HIP kernels.
+Spin-1/2 chains and the spin-1/2 full-space Quantum Sun return matrix-free
+device operators: the basis, connectivity and matrix elements are computed by
+the HIP kernels.  The other Quantum-Sun variants (spin S, the U(1) sector)
+build their sparse matrix on the host, as the reference does, and run on the
+packed operator.
 Random fields/disorder use a numpy ``Generator`` (Julia's RNG streams cannot be
 reproduced); explicit arrays reproduce any realization exactly.
 """
@@ -11,7 +15,9 @@ from math import sqrt
 
 import numpy as np
 
-from .operators import QSunOperator, SpinChainOperator
+import scipy.sparse as sp
+
+from .operators import Operator, PackedOperator, QSunOperator, SpinChainOperator
 
 SUPPORTED_BOUNDARIES = ("open", "periodic")
 
@@ -95,16 +101,138 @@ def qsun_draws(L_loc: int, L_grain: int, alpha: float, *, gamma: float = 1.0, w:
     return disorder, neighbors, couplings, H_grain
 
 
+def qsun_spin_draws(L_loc: int, L_grain: int, alpha: float, d: int, *, w: float = 0.5,
+                    hz: float = 1.0, zeta: float = 0.2, rng=None):
+    """Disorder, neighbors, couplings and the raw GOE grain matrix
+    ``(X + X^T)/sqrt(2)`` of dimension d^L_grain, in the reference's draw order
+    (QuantumSun.jl:21-28, QuantumSun_U1_conserving.jl:23-31)."""
+    rng = rng if rng is not None else np.random.default_rng()
+    disorder = rng.random(L_loc) * (2 * w) + (hz - w)
+    neighbors = rng.integers(0, L_grain, size=L_loc)
+    couplings = np.zeros(L_loc)
+    if alpha > 0:
+        couplings[0] = 1.0
+        for j in range(2, L_loc + 1):
+            couplings[j - 1] = alpha ** (j - 1 + rng.random() * 2 * zeta - zeta)
+    n = d ** L_grain
+    X = rng.standard_normal((n, n))
+    return disorder, neighbors, couplings, (X + X.T) / sqrt(2)
+
+
+def qsun_spin_matrix(L_loc: int, L_grain: int, g0: float, alpha: float, *, gamma: float = 1.0,
+                     w: float = 0.5, hz: float = 1.0, zeta: float = 0.2, S: float = 0.5,
+                     use_U1: bool = False, S_z: float = 0.0, rng=None, draws=None):
+    """The Quantum Sun as a scipy CSR matrix for spin S: the full space
+    (``build_hamiltonian``, src/QSun/QuantumSun.jl:1-102: S^x S^x outer-grain
+    couplings, grain gamma*GOE/sqrt(d^Lg + 1)) or, with ``use_U1``, the sector
+    of total S^z = ``S_z`` (``build_hamiltonian_cons``,
+    QuantumSun_U1_conserving.jl:1-212: S^x S^x + S^y S^y couplings, grain
+    entries between different grain magnetizations removed and the grain part
+    normalized to gamma * sqrt(tr(R^2)/dim) = gamma).
+
+    Basis (src/QSun/HilbertSpace.jl:11-41, SpinOperators.jl:5-31): site k
+    holds the digit m_k + S in [0, d), d = 2S+1, and the states sort like the
+    mixed-radix number sum_k digit_k d^k, so that is the row index (in the
+    sector, the rank among the sector's states).  ``draws`` =
+    (disorder, neighbors, couplings, raw GOE) as from :func:`qsun_spin_draws`.
+    """
+    if (2 * S) != int(2 * S) or S <= 0:
+        raise ValueError(f"Wrong value of spin. Only integer spins or half-integer spins are possible. You chose S = {S}")
+    d = int(round(2 * S)) + 1
+    L = L_loc + L_grain
+    if use_U1 and (abs(S_z) > L * S or (S_z + L * S) != int(S_z + L * S)):
+        raise ValueError(f"Wrong total value of spin to given spin-S particles. You chose S = {S} and S_z = {S_z}.")
+    if draws is None:
+        draws = qsun_spin_draws(L_loc, L_grain, alpha, d, w=w, hz=hz, zeta=zeta, rng=rng)
+    disorder, neighbors, couplings, goe = draws
+    N = d ** L
+    idx = np.arange(N, dtype=np.int64)
+    pw = d ** np.arange(L, dtype=np.int64)
+    m = (idx[:, None] // pw[None, :]) % d - S            # (N, L) spin values
+    if use_U1:
+        keep = m.sum(axis=1) == S_z
+        rank = np.full(N, -1, dtype=np.int64)
+        rank[keep] = np.arange(int(keep.sum()))
+    else:
+        keep = np.ones(N, dtype=bool)
+        rank = idx
+    dim = int(keep.sum())
+    rows, cols, vals = [], [], []
+    k = idx[keep]
+    # fields on the outer spins: sum_j h_j S^z
+    rows.append(rank[k])
+    cols.append(rank[k])
+    vals.append((m[k][:, L_grain:] * disorder[None, :]).sum(axis=1))
+
+    def ladder(mm, up):  # S^+ / S^- amplitude sqrt(S(S+1) - m(m +- 1))
+        return np.sqrt(np.maximum(S * (S + 1) - mm * (mm + (1 if up else -1)), 0.0))
+
+    for j in range(L_loc):
+        sl, sg = L_grain + j, int(neighbors[j])
+        ml, mg = m[k, sl], m[k, sg]
+        for ul in (True, False):
+            okl = ml < S if ul else ml > -S
+            for ug in (True, False):
+                okg = mg < S if ug else mg > -S
+                ok = okl & okg
+                # S^x S^x: (1/2)(1/2) amplitudes; + S^y S^y in the U(1) form:
+                # (1/2i)(1/2i) sign_l sign_g, i.e. -(1/4) sign_l sign_g
+                coef = 0.25 if not use_U1 else 0.25 * (1 - (1 if ul == ug else -1))
+                if coef == 0.0:
+                    continue
+                new = k + (pw[sl] if ul else -pw[sl]) + (pw[sg] if ug else -pw[sg])
+                ok &= rank[np.where(ok, new, 0)] >= 0
+                amp = ladder(ml, ul) * ladder(mg, ug)
+                rows.append(rank[new[ok]])
+                cols.append(rank[k[ok]])
+                vals.append(g0 * couplings[j] * coef * amp[ok])
+    H = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(dim, dim)).tocsr()
+    # grain: kron(I_loc, G) on the low L_grain sites
+    n = d ** L_grain
+    G = np.array(goe, dtype=np.float64, copy=True)
+    if use_U1:
+        gm = ((np.arange(n)[:, None] // pw[None, :L_grain]) % d - S).sum(axis=1)
+        G[gm[:, None] != gm[None, :]] = 0.0
+    else:
+        G = gamma * G / sqrt(n + 1)
+    a = idx[keep] % n
+    blk = idx[keep] // n
+    # pairs of sector states in the same outer configuration (blk) and any grain states
+    order = np.argsort(blk, kind="stable")
+    blk_s, a_s, r_s = blk[order], a[order], rank[idx[keep]][order]
+    bounds = np.flatnonzero(np.diff(np.r_[-1, blk_s, -2]))
+    gr, gc, gv = [], [], []
+    for lo, hi in zip(bounds[:-1], bounds[1:]):
+        aa, rr = a_s[lo:hi], r_s[lo:hi]
+        vv = G[np.ix_(aa, aa)]
+        nz = vv != 0.0
+        ii, jj = np.nonzero(nz)
+        gr.append(rr[ii])
+        gc.append(rr[jj])
+        gv.append(vv[nz])
+    R = sp.coo_matrix((np.concatenate(gv), (np.concatenate(gr), np.concatenate(gc))),
+                      shape=(dim, dim)).tocsr()
+    if use_U1:
+        R = gamma * R / sqrt((R.multiply(R)).sum() / dim)
+    return (H + R).tocsr()
+
+
 def qsun_hamiltonian(L_loc: int, L_grain: int, g0: float, alpha: float, *, gamma: float = 1.0,
                      w: float = 0.5, hz: float = 1.0, zeta: float = 0.2, S: float = 0.5,
-                     rng=None, use_U1: bool = False, draws=None, device=None) -> QSunOperator:
-    """``qsun_hamiltonian`` (src/QSun/QSun.jl:37-84) for the full space, S = 1/2.
-    ``draws`` = (disorder, neighbors, couplings, H_grain) reproduces a given
-    realization."""
+                     rng=None, use_U1: bool = False, S_z: float = 0.0, draws=None,
+                     device=None) -> Operator:
+    """``qsun_hamiltonian`` (src/QSun/QSun.jl:37-84).  The spin-1/2 full space
+    is the matrix-free Quantum-Sun operator (``draws`` = (disorder, neighbors,
+    couplings, H_grain) reproduces a realization); spin S and the U(1) sector
+    (``use_U1``, total S^z = ``S_z``) are built by :func:`qsun_spin_matrix`
+    and run on the packed operator (``draws`` then holds the raw GOE)."""
     if (2 * S) != int(2 * S):
         raise ValueError(f"Wrong value of spin. Only integer spins or half-integer spins are possible. You chose S = {S}")
     if S != 0.5 or use_U1:
-        raise NotImplementedError("the HIP Quantum Sun operator covers the full space of spin 1/2")
+        H = qsun_spin_matrix(L_loc, L_grain, g0, alpha, gamma=gamma, w=w, hz=hz, zeta=zeta, S=S,
+                             use_U1=use_U1, S_z=S_z, rng=rng, draws=draws)
+        return PackedOperator.from_matrix(H, device=device)
     if draws is None:
         draws = qsun_draws(L_loc, L_grain, alpha, gamma=gamma, w=w, hz=hz, zeta=zeta, rng=rng)
     disorder, neighbors, couplings, H_grain = draws

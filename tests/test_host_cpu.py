@@ -176,3 +176,81 @@ def test_pack_by_value_matches_oracle_on_random_matrices():
     assert len(vv) == len(buckets)
     for (v, f, s), (rv, rf, rs) in zip(_per_value_1based(vv, starts, flat), buckets):
         assert v == rv and f == list(rf + 1) and s == list(rs + 1)
+
+
+# ------------------------------------------------ Quantum Sun, spin S and U(1)
+def _kron_qsun(L_loc, L_grain, g0, S, use_U1, S_z, gamma, draws):
+    """Independent dense construction from textbook spin-S matrices and
+    Kronecker products (site 0 = the last, fastest factor)."""
+    disorder, neighbors, couplings, goe = draws
+    d = int(round(2 * S)) + 1
+    L = L_loc + L_grain
+    ms = np.arange(d) - S
+    Sp = np.zeros((d, d))
+    for k in range(d - 1):
+        Sp[k + 1, k] = np.sqrt(S * (S + 1) - ms[k] * (ms[k] + 1))
+    ops = {"z": np.diag(ms), "x": (Sp + Sp.T) / 2, "y": (Sp - Sp.T) / 2j}
+
+    def at(name, site):
+        out = np.ones((1, 1))
+        for s in reversed(range(L)):
+            out = np.kron(out, ops[name] if s == site else np.eye(d))
+        return out
+
+    N = d ** L
+    H = np.zeros((N, N), dtype=complex)
+    for j in range(L_loc):
+        H += disorder[j] * at("z", L_grain + j)
+        H += g0 * couplings[j] * at("x", L_grain + j) @ at("x", int(neighbors[j]))
+        if use_U1:
+            H += g0 * couplings[j] * at("y", L_grain + j) @ at("y", int(neighbors[j]))
+    n = d ** L_grain
+    G = np.array(goe, dtype=float)
+    if use_U1:
+        gm = np.array([sum((a // d ** s) % d - S for s in range(L_grain)) for a in range(n)])
+        G[gm[:, None] != gm[None, :]] = 0.0
+        mag = np.array([sum((i // d ** s) % d - S for s in range(L)) for i in range(N)])
+        sel = np.flatnonzero(mag == S_z)
+        R = np.kron(np.eye(d ** L_loc), G)[np.ix_(sel, sel)]
+        R = gamma * R / np.sqrt(np.trace(R @ R) / len(sel))
+        H = H[np.ix_(sel, sel)] + R
+    else:
+        H = H + np.kron(np.eye(d ** L_loc), gamma * G / np.sqrt(n + 1))
+    assert np.abs(H.imag).max() < 1e-14
+    return H.real
+
+
+@pytest.mark.parametrize("L_loc,L_grain,S,use_U1,S_z", [
+    (3, 2, 0.5, True, 0.5), (2, 2, 1.0, True, 0.0), (2, 1, 1.0, False, 0.0),
+    (2, 1, 1.5, True, 0.5), (3, 2, 0.5, False, 0.0), (1, 2, 1.0, True, -1.0)])
+def test_qsun_spin_matrix_equals_kronecker_construction(L_loc, L_grain, S, use_U1, S_z):
+    """qsun_spin_matrix (bit-encoded basis, ladder amplitudes, sector ranks)
+    against the dense tensor-product Hamiltonian restricted to the sector."""
+    from polfed_amd import models
+    d = int(round(2 * S)) + 1
+    draws = models.qsun_spin_draws(L_loc, L_grain, 0.6, d, rng=np.random.default_rng(7))
+    H = models.qsun_spin_matrix(L_loc, L_grain, 0.9, 0.6, S=S, use_U1=use_U1, S_z=S_z,
+                                gamma=1.3, draws=draws)
+    ref = _kron_qsun(L_loc, L_grain, 0.9, S, use_U1, S_z, 1.3, draws)
+    assert H.shape == ref.shape
+    assert np.abs(H.toarray() - ref).max() < 1e-13
+
+
+def test_qsun_spin_matrix_reference_shapes_and_spin_half_oracle():
+    """The sector sizes of test/test_qsun.jl:20-70, and the spin-1/2 full space
+    equal to the oracle's Quantum Sun for the same draws."""
+    from oracle import models as om
+    from polfed_amd import models
+    for kw, n in [(dict(L_loc=2, L_grain=2, use_U1=True, S_z=0.0), 6),
+                  (dict(L_loc=2, L_grain=1, use_U1=True, S_z=0.5), 3),
+                  (dict(L_loc=1, L_grain=1, S=1.5, use_U1=True, S_z=1.0), 3),
+                  (dict(L_loc=2, L_grain=2), 16)]:
+        H = models.qsun_spin_matrix(g0=1.0, alpha=0.5, rng=np.random.default_rng(3), **kw)
+        assert H.shape == (n, n) and abs(H - H.T).max() == 0.0
+    with pytest.raises(ValueError):
+        models.qsun_spin_matrix(2, 2, 1.0, 0.5, use_U1=True, S_z=0.5)  # not a sector
+    draws = models.qsun_spin_draws(5, 3, 0.75, 2, rng=np.random.default_rng(4))
+    H = models.qsun_spin_matrix(5, 3, 1.0, 0.75, draws=draws)
+    dis, nb, cp, goe = draws
+    ref = om.qsun_hamiltonian_from_draws(5, 3, 1.0, dis, nb, cp, goe / np.sqrt(9))
+    assert abs(H - ref).max() < 1e-14

@@ -173,6 +173,55 @@ def filter_only(args):
                       "algorithmic_GBps": 4 * 8 * op.D * esz / (us * 1e-6) / 1e9}), flush=True)
 
 
+def packed_compare(args):
+    """Profiling driver: the reference's own GPU data structure -- the sparse
+    matrix packed by value (``polfed(mat, ...)``) -- against the matrix-free
+    kernel, on the same XXZ chains (L=20 FP64 block 8, L=22 FP32 block 16):
+    the matrix is read back from the matrix-free operator's connectivity,
+    packed and uploaded; both are autotuned and time 3 order-200 filters.
+    Prints one JSON line per (chain, operator)."""
+    import numpy as np
+    import scipy.sparse as sp
+    import torch
+    from polfed_amd import models
+    from polfed_amd.device import Device
+    from polfed_amd.operators import PackedOperator
+    dev = Device(0)
+    for L, s, dt in ((20, 8, "f64"), (22, 16, "f32")):
+        mf = models.xxz_hamiltonian(L, L // 2, 1.0, 1.0, 3.0, rng=np.random.default_rng(1),
+                                    device=dev)
+        D = mf.D
+        cols, vals, counts, diag = mf.connectivity(max_per_row=64)
+        keep = np.arange(64)[None, :] < counts[:, None]
+        H = (sp.coo_matrix((vals[keep], (np.repeat(np.arange(D), counts), cols[keep])),
+                           shape=(D, D)) + sp.diags(diag)).tocsr()
+        pk = PackedOperator.from_matrix(H, device=dev)
+        X = dev.to_device(np.random.default_rng(0).standard_normal((D, s)))
+        c = np.cos(np.arange(201) * 0.3) * 0.01
+        ys = {}
+        for name, op in (("matrix-free", mf), ("packed", pk)):
+            op.set_rescale(-20.0, 20.0)
+            op.autotune(s, dt, 30)
+            Y = op.filter(c, X, dtype=dt)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(3):
+                op.filter(c, X, Y, dtype=dt)
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) * 1e3 / (3 * 201)
+            ys[name] = op.filter(c, X, dtype="f64")
+            esz = 8 if dt == "f64" else 4
+            print(json.dumps({"mode": "packed-compare", "L": L, "D": D, "s": s, "dtype": dt,
+                              "operator": name, "nnz_offdiag": int(H.nnz - D),
+                              "us_per_step": us,
+                              "algorithmic_GBps": 4 * s * D * esz / (us * 1e-6) / 1e9}), flush=True)
+        torch.cuda.synchronize()
+        diff = float((ys["packed"] - ys["matrix-free"]).abs().max() / ys["matrix-free"].abs().max())
+        print(json.dumps({"mode": "packed-compare", "L": L, "fp64_filter_rel_diff": diff}), flush=True)
+
+
 def reorth_only(args):
     """Profiling driver: CGS2 reorthogonalization W -= V (V^T W), twice, of an
     8-column block against k=416 basis vectors at D=184756 (the basis size the
@@ -488,6 +537,8 @@ def main():
                     help="profiling driver: time about STEPS filter steps only")
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"],
                     help="filter arithmetic for --filter-only")
+    ap.add_argument("--packed-compare", action="store_true",
+                    help="profiling driver: packed sparse operator vs matrix-free filter steps")
     ap.add_argument("--layout", default="", metavar="V,PBLOCKS,WIDE,PARTITION",
                     help="--filter-only: fixed step-kernel launch configuration instead of "
                          "the autotuner's (e.g. 2,8,0,1)")
@@ -519,6 +570,8 @@ def main():
         return filter_only(args)
     if args.reorth_only:
         return reorth_only(args)
+    if args.packed_compare:
+        return packed_compare(args)
     if args.workload == "qsun-batch":
         return qsun_batch(args)
     if args.workload == "xxz-l22":

@@ -26,24 +26,6 @@
 namespace pf {
 
 // ------------------------------------------------------ spin-chain kernel
-// Broadcast a 32-bit value from lane Q of this lane's group of G lanes (the
-// G lanes that share one row).  G <= 4 uses a DPP quad permutation, G = 8 a
-// cross-lane permute.
-template <int G, int Q>
-__device__ __forceinline__ uint32_t group_bcast(uint32_t v) {
-  if constexpr (G <= 1) {
-    return v;
-  } else if constexpr (G == 2) {
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, Q | (Q << 2) | ((Q + 2) << 4) | ((Q + 2) << 6),
-                                              0xF, 0xF, false);
-  } else if constexpr (G == 4) {
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, Q * 0x55, 0xF, 0xF, false);
-  } else {
-    const int lane = threadIdx.x & 63;
-    return (uint32_t)__shfl((int)v, (lane & ~(G - 1)) + Q, 64);
-  }
-}
-
 // Where a lane finds neighbour ranks: the two rank tables staged in LDS
 // (TBL) or the O(1) update from the binomial table; bond masks in LDS.
 struct RankCtx {
@@ -88,23 +70,7 @@ __device__ __forceinline__ void bond_ranks(const RankCtx& R, int b0, uint32_t st
     uint32_t mine[UB];
 #pragma unroll
     for (int u = 0; u < UB; ++u) mine[u] = one_rank<TBL>(R, b0 + g + GM * u, st, r);
-#pragma unroll
-    for (int u = 0; u < UB; ++u) {
-#pragma unroll
-      for (int q = 0; q < GM; ++q) {
-        uint32_t v;
-        if constexpr (GM == 1) v = mine[u];
-        else if (q == 0) v = group_bcast<GM, 0>(mine[u]);
-        else if (q == 1) v = group_bcast<GM, 1>(mine[u]);
-        else if (q == 2) v = group_bcast<GM, 2>(mine[u]);
-        else if (q == 3) v = group_bcast<GM, 3>(mine[u]);
-        else if (q == 4) v = group_bcast<GM, 4>(mine[u]);
-        else if (q == 5) v = group_bcast<GM, 5>(mine[u]);
-        else if (q == 6) v = group_bcast<GM, 6>(mine[u]);
-        else v = group_bcast<GM, 7>(mine[u]);
-        out[q + GM * u] = v;
-      }
-    }
+    group_share<GM, UB>(mine, out);
   }
 }
 

@@ -26,11 +26,13 @@ namespace pf {
 
 // out = cx*X + ch*((H - shift)*scale) b2 + cb*b3 for a packed operator.  The
 // G = s/V lanes of a row each gather their V columns of every entry; NB
-// entries are loaded before any is accumulated.  P2: the row stride in bytes
-// is a power of two (offsets by shift).  Each entry adds (v*scale) * x_j, so
-// the terms of one value are not summed before the multiply as in the
-// reference (a rounding-level difference).
-template <typename T, int V, bool P2>
+// entries are loaded before any is accumulated.  With GM = G in {2, 4, 8} the
+// lanes of a row load different entries of a batch and share them by
+// cross-lane broadcasts (GM = 1 or 0: every lane loads all of them).  P2: the
+// row stride in bytes is a power of two (offsets by shift).  Each entry adds
+// (v*scale) * x_j, so the terms of one value are not summed before the
+// multiply as in the reference (a rounding-level difference).
+template <typename T, int V, bool P2, int GM>
 __global__ __launch_bounds__(256) void packed_step_kernel(
     const int64_t D, const int64_t* __restrict__ prow, const uint32_t* __restrict__ pent,
     const double* __restrict__ diag, const double* __restrict__ pvals, const int nvals,
@@ -39,7 +41,8 @@ __global__ __launch_bounds__(256) void packed_step_kernel(
     void* __restrict__ OUT, const Layout lay, const int64_t out_ld,
     double* __restrict__ dot_part, const T* __restrict__ Pr, const int64_t ngroups) {
   typedef typename VT<T, V>::t vt;
-  constexpr int NB = (V * (int)sizeof(T) <= 32) ? 8 : 4;  // gathers in flight
+  constexpr int NB0 = (V * (int)sizeof(T) <= 32) ? 8 : 4;  // gathers in flight
+  constexpr int NB = NB0 < GM ? GM : NB0;
   __shared__ T sV[kMaxPackedVals];
   __shared__ double red[4];
   for (int i = threadIdx.x; i < nvals; i += blockDim.x) sV[i] = (T)(pvals[i] * af.scale);
@@ -77,8 +80,19 @@ __global__ __launch_bounds__(256) void packed_step_kernel(
       vt tot = (vt)(T)0;
       for (int64_t e = e0; e < e1; e += NB) {
         uint32_t en[NB];
+        if constexpr (GM >= 2) {
+          constexpr int UB = NB / GM;
+          uint32_t mine[UB];
 #pragma unroll
-        for (int u = 0; u < NB; ++u) en[u] = e + u < e1 ? pent[e + u] : 0u;
+          for (int u = 0; u < UB; ++u) {
+            const int64_t ee = e + g + GM * u;
+            mine[u] = ee < e1 ? pent[ee] : 0u;
+          }
+          group_share<GM, UB>(mine, en);
+        } else {
+#pragma unroll
+          for (int u = 0; u < NB; ++u) en[u] = e + u < e1 ? pent[e + u] : 0u;
+        }
         vt x[NB];
         T w[NB];
 #pragma unroll
@@ -122,14 +136,25 @@ static void launch_packed_typed(pf_op* op, int flags, const Layout& lay, int nco
   if (lay.nt) flags |= kStreamNT;
   const uint32_t rsb = (uint32_t)(lay.rs * sizeof(T));
   const bool p2 = (rsb & (rsb - 1u)) == 0;
-#define PF_PACKED_LAUNCH(P2V)                                                                    \
-  hipLaunchKernelGGL((packed_step_kernel<T, V, P2V>), grid, block, 0, st, op->D, op->prow,        \
+#define PF_PACKED_LAUNCH(P2V, GMV)                                                               \
+  hipLaunchKernelGGL((packed_step_kernel<T, V, P2V, GMV>), grid, block, 0, st, op->D, op->prow,  \
                      op->pent, op->diag, op->pvals, op->pnvals, af, flags, (T)cx, (T)ch, (T)cb,   \
                      reinterpret_cast<const T*>(X), reinterpret_cast<const T*>(B2),               \
                      reinterpret_cast<const T*>(B3), out, lay, out_ld, dot_part,                  \
                      reinterpret_cast<const T*>(P), ngroups)
-  if (p2) PF_PACKED_LAUNCH(true);
-  else PF_PACKED_LAUNCH(false);
+#define PF_PACKED_G(P2V)                          \
+  switch (lay.G) {                                \
+    case 2: PF_PACKED_LAUNCH(P2V, 2); break;      \
+    case 4: PF_PACKED_LAUNCH(P2V, 4); break;      \
+    case 8: PF_PACKED_LAUNCH(P2V, 8); break;      \
+    default: PF_PACKED_LAUNCH(P2V, 1); break;     \
+  }
+  if (p2) {
+    PF_PACKED_G(true);
+  } else {
+    PF_PACKED_G(false);
+  }
+#undef PF_PACKED_G
 #undef PF_PACKED_LAUNCH
   PF_HIP(hipGetLastError());
 }

@@ -75,6 +75,48 @@ __device__ __forceinline__ T vget(typename VT<T, V>::t v, int i) {
   }
 }
 
+// Broadcast a 32-bit value from lane Q of this lane's group of G lanes (the
+// G lanes that share one row).  G <= 4 uses a DPP quad permutation, G = 8 a
+// cross-lane permute.
+template <int G, int Q>
+__device__ __forceinline__ uint32_t group_bcast(uint32_t v) {
+  if constexpr (G <= 1) {
+    return v;
+  } else if constexpr (G == 2) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, Q | (Q << 2) | ((Q + 2) << 4) | ((Q + 2) << 6),
+                                              0xF, 0xF, false);
+  } else if constexpr (G == 4) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, Q * 0x55, 0xF, 0xF, false);
+  } else {
+    const int lane = threadIdx.x & 63;
+    return (uint32_t)__shfl((int)v, (lane & ~(G - 1)) + Q, 64);
+  }
+}
+
+// The G = GM lanes of a row group each hold UB values (lane q's u-th value is
+// item q + GM*u); afterwards every lane of the group holds all GM*UB items.
+template <int GM, int UB>
+__device__ __forceinline__ void group_share(const uint32_t (&mine)[UB],
+                                            uint32_t (&out)[GM * UB]) {
+#pragma unroll
+  for (int u = 0; u < UB; ++u) {
+#pragma unroll
+    for (int q = 0; q < GM; ++q) {
+      uint32_t v;
+      if constexpr (GM == 1) v = mine[u];
+      else if (q == 0) v = group_bcast<GM, 0>(mine[u]);
+      else if (q == 1) v = group_bcast<GM, 1>(mine[u]);
+      else if (q == 2) v = group_bcast<GM, 2>(mine[u]);
+      else if (q == 3) v = group_bcast<GM, 3>(mine[u]);
+      else if (q == 4) v = group_bcast<GM, 4>(mine[u]);
+      else if (q == 5) v = group_bcast<GM, 5>(mine[u]);
+      else if (q == 6) v = group_bcast<GM, 6>(mine[u]);
+      else v = group_bcast<GM, 7>(mine[u]);
+      out[q + GM * u] = v;
+    }
+  }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs; this bijection gives
 // XCD x a contiguous range of row groups, so the neighbour gathers of nearby
 // rows hit that XCD's own L2.

@@ -137,26 +137,37 @@ def measured_traffic(kernel):
 
 
 def filter_only(args):
-    """Profiling driver: the bench workload's operator and start block, the
-    autotuned step kernel, `--filter-only` Clenshaw steps of a fixed order-200
-    filter; prints one JSON line with the average step time (HIP events)."""
+    """Profiling driver: `--filter-only` Clenshaw steps of a fixed filter on the
+    bench workload's operator and start block (XXZ L=20, FP64 block 8, order
+    200), or with ``--workload j1j2-l24`` on the J1-J2 L=24 operator with a
+    random 32-column block (order 20); the autotuned step kernel unless
+    ``--layout`` fixes it.  Prints one JSON line with the average step time
+    (HIP events)."""
     import numpy as np
     import torch
+    from polfed_amd import models
     from polfed_amd.device import Device
     dev = Device(0)
-    op, x0 = build_workload(dev)
-    op.set_rescale(-17.2, 13.5)
+    if args.workload == "j1j2-l24":
+        op = models.j1j2_hamiltonian(24, 12, 1.0, 0.5, 1.0, 1.0, 0.0, device=dev)
+        op.set_rescale(-9.0, 9.0)
+        s, K = 32, 20
+        x0 = dev.to_device(np.random.default_rng(0).standard_normal((op.D, s)))
+    else:
+        op, x0 = build_workload(dev)
+        op.set_rescale(-17.2, 13.5)
+        s, K = 8, 200
     dt = args.dtype
     if args.layout:  # a fixed launch configuration (no autotuner trials in the profile)
         V, pb, wide, part = (int(v) for v in args.layout.split(","))
         op.set_tuning(vec=V, pblocks=pb, wide=wide, row_partition=part)
     else:
-        op.ensure_tuned(8, dt)
+        op.ensure_tuned(s, dt)
     if args.row_partition >= 0:  # force the row order over the autotuned choice
         op.set_tuning(row_partition=args.row_partition)
-    c = np.cos(np.arange(201) * 0.3) * 0.01
+    c = np.cos(np.arange(K + 1) * 0.3) * 0.01
     y = torch.empty_like(x0)
-    reps = max(1, args.filter_only // 201)
+    reps = max(1, args.filter_only // (K + 1))
     op.filter(c, x0, y, dtype=dt)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -165,12 +176,12 @@ def filter_only(args):
         op.filter(c, x0, y, dtype=dt)
     e1.record()
     torch.cuda.synchronize()
-    us = e0.elapsed_time(e1) * 1e3 / (reps * 201)
+    us = e0.elapsed_time(e1) * 1e3 / (reps * (K + 1))
     esz = 8 if dt == "f64" else 4
-    print(json.dumps({"mode": "filter-only", "kernel": kernel_name(op, dt),
+    print(json.dumps({"mode": "filter-only", "kernel": kernel_name(op, dt, s),
                       "layout": args.layout or "autotuned", "dtype": dt,
-                      "D": op.D, "s": 8, "steps": reps * 201, "us_per_step": us,
-                      "algorithmic_GBps": 4 * 8 * op.D * esz / (us * 1e-6) / 1e9}), flush=True)
+                      "D": op.D, "s": s, "steps": reps * (K + 1), "us_per_step": us,
+                      "algorithmic_GBps": 4 * s * op.D * esz / (us * 1e-6) / 1e9}), flush=True)
 
 
 def packed_compare(args):

@@ -97,10 +97,11 @@ int pf_op_create_qsun(pf_handle* h, int32_t L_grain, int32_t L_loc, double g0,
  * columns flat[starts[v*(D+1)+r] .. starts[v*(D+1)+r+1]) (0-based columns,
  * absolute offsets into the concatenated flat array, one (D+1)-entry start
  * table per value; the reference keeps one 1-based flat/starts pair per
- * value).  Host arrays.  nvals <= 256; more than one value needs D <= 2^24. */
+ * value); nflat = length of flat (every start must lie in [0, nflat]).  Host
+ * arrays.  nvals <= 256; more than one value needs D <= 2^24. */
 int pf_op_create_packed(pf_handle* h, int64_t D, const double* diag, int32_t nvals,
                         const double* vals, const int64_t* starts, const int32_t* flat,
-                        pf_op** out);
+                        int64_t nflat, pf_op** out);
 
 int pf_op_destroy(pf_op* op);
 

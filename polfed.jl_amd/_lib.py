@@ -82,7 +82,7 @@ SIGNATURES = {
     "pf_op_create_qsun": (_c.c_int, [_vp, _i32, _i32, _dbl, _pdbl, _pi32, _pdbl, _pdbl,
                                      _c.POINTER(_vp)]),
     "pf_op_create_packed": (_c.c_int, [_vp, _i64, _pdbl, _i32, _pdbl, _c.POINTER(_i64), _pi32,
-                                       _c.POINTER(_vp)]),
+                                       _i64, _c.POINTER(_vp)]),
     "pf_op_destroy": (_c.c_int, [_vp]),
     "pf_op_dim": (_i64, [_vp]),
     "pf_op_set_rescale": (_c.c_int, [_vp, _dbl, _dbl]),

@@ -232,12 +232,13 @@ using namespace pf;
 
 extern "C" int pf_op_create_packed(pf_handle* h, int64_t D, const double* diag, int32_t nvals,
                                    const double* vals, const int64_t* starts,
-                                   const int32_t* flat, pf_op** out) {
+                                   const int32_t* flat, int64_t nflat, pf_op** out) {
   return guarded([&] {
     PF_REQUIRE(h && out && diag, "packed: null argument");
     PF_REQUIRE(D >= 1 && D <= (int64_t)0xFFFFFFFF, "packed: 1 <= D < 2^32");
     PF_REQUIRE(nvals >= 0 && nvals <= kMaxPackedVals, "packed: at most 256 distinct values");
-    PF_REQUIRE(nvals == 0 || (vals && starts && flat), "packed: null value arrays");
+    PF_REQUIRE(nvals == 0 || (vals && starts && (flat || nflat == 0)), "packed: null value arrays");
+    PF_REQUIRE(nflat >= 0, "packed: nflat >= 0");
     PF_REQUIRE(nvals <= 1 || D <= ((int64_t)1 << 24),
                "packed: more than one distinct value needs D <= 2^24");
     *out = nullptr;
@@ -245,7 +246,7 @@ extern "C" int pf_op_create_packed(pf_handle* h, int64_t D, const double* diag, 
     std::vector<int64_t> rowptr(D + 1, 0);
     for (int v = 0; v < nvals; ++v) {
       const int64_t* sv = starts + (int64_t)v * (D + 1);
-      PF_REQUIRE(sv[0] >= 0, "packed: negative start");
+      PF_REQUIRE(sv[0] >= 0 && sv[D] <= nflat, "packed: starts outside flat[0, nflat)");
       for (int64_t r = 0; r < D; ++r) {
         PF_REQUIRE(sv[r + 1] >= sv[r], "packed: starts must be non-decreasing");
         rowptr[r + 1] += sv[r + 1] - sv[r];

@@ -262,7 +262,7 @@ class PackedOperator(Operator):
         i64p = st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
         with torch.cuda.device(device.index):
             check(device.lib.pf_op_create_packed(device.h, D, dptr(dg), vv.size, dptr(vv), i64p,
-                                                 iptr(fl), ctypes.byref(op)))
+                                                 iptr(fl), fl.size, ctypes.byref(op)))
         super().__init__(device, op)
         self.diag, self.vals, self.starts, self.flat = dg, vv, st, fl
 

@@ -190,3 +190,22 @@ def test_polfed_on_qsun_spin_and_u1_sectors(dev, kw):
     from polfed_amd.packing import pack_by_value
     Hp = _packed_matrix(*pack_by_value(H))
     assert np.linalg.norm(Hp @ psi - psi * vals, axis=0).max() <= 1e-10
+
+
+def test_packed_rejects_inconsistent_arrays(dev):
+    """The C ABI checks every value's start table against the flat length and
+    every column against D (no host read past the caller's arrays)."""
+    import ctypes
+    from polfed_amd import _lib
+    from polfed_amd.packing import pack_by_value
+    diag, vals, starts, flat = pack_by_value(MATS["j1j2_L12"])
+    lib = dev.lib
+    i64p = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    out = ctypes.c_void_p()
+    D = diag.size
+    for fl, n in [(flat, flat.size - 1), (np.where(flat == 0, D, flat).astype(np.int32), flat.size)]:
+        fl = np.ascontiguousarray(fl, dtype=np.int32)
+        rc = lib.pf_op_create_packed(dev.h, D, _lib.dptr(diag), vals.size, _lib.dptr(vals),
+                                     i64p(np.ascontiguousarray(starts)), _lib.iptr(fl), n,
+                                     ctypes.byref(out))
+        assert rc != 0 and not out.value

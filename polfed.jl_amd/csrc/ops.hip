@@ -1388,7 +1388,9 @@ extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nste
     };
     std::vector<Cand> cands;
     const int vs[6] = {1, 2, 4, 8, 16, 32};
-    const int pbs[3] = {default_pblocks(op), 4, 0};
+    // persistent blocks per CU: beyond the resident count (12, 16) the extra
+    // blocks start as others finish, which balances the last row groups
+    const int pbs[5] = {default_pblocks(op), 4, 0, 12, 16};
     for (int vi = 0; vi < 6; ++vi) {
       if (!valid_vec(dtype, ncols, vs[vi])) continue;
       // kernel variants: for spin chains with rank tables the plain step and
@@ -1397,7 +1399,7 @@ extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nste
       // 8-waves-per-SIMD build is left to forced tuning: it never won)
       const bool ctab = op->kind == kChain && op->chain.tsplit > 0;
       const int nvar = ctab ? (op->pstates ? 4 : 2) : (op->kind == kQsun ? 2 : 1);
-      for (int pi = 0; pi < 3; ++pi) {
+      for (int pi = 0; pi < 5; ++pi) {
         if (pi > 0 && pbs[pi] == pbs[0]) continue;
         for (int w = 0; w < nvar; ++w) {
           TuneCfg c;

@@ -521,9 +521,10 @@ def cpu_baseline(op, report, x0_host, budget_s=15.0):
     threads = min(s, os.cpu_count() or 1)
     c = spc.normalized_coefficients(spc.PolfedDefaults.coefficients, report.target, report.order)
     steps_app = report.order + 1
+    P.filter(c[:51], x0_host, nthreads=threads)  # warm-up (threads, first touch)
     t0 = time.perf_counter()
-    P.filter(c[:51], x0_host, nthreads=threads)  # estimate on a 51-step prefix
-    est_app = (time.perf_counter() - t0) / 51 * steps_app
+    P.filter(c[:201], x0_host, nthreads=threads)  # estimate on a 201-step prefix
+    est_app = (time.perf_counter() - t0) / 201 * steps_app
     if est_app <= budget_s:
         napps = max(1, min(4, int(np.ceil(budget_s / est_app))))
         t0 = time.perf_counter()
@@ -533,7 +534,7 @@ def cpu_baseline(op, report, x0_host, budget_s=15.0):
         nsteps = napps * steps_app
         what = f"{napps} whole filter application(s) ({napps} x {steps_app} steps)"
     else:
-        nsteps = int(max(51, budget_s / est_app * steps_app))
+        nsteps = int(max(201, budget_s / est_app * steps_app))
         t0 = time.perf_counter()
         P.filter(c[:nsteps], x0_host, nthreads=threads)
         t = time.perf_counter() - t0

@@ -334,3 +334,46 @@ def test_residuals_and_coldots(dev):
     assert np.max(np.abs(res - rref) / rref) < 1e-6
     d = bo.coldots(dev, dev.to_device(psi), dev.to_device(H @ psi))
     assert np.max(np.abs(d - Eref)) < 1e-12
+
+
+@pytest.mark.parametrize("L,Lup,bc", [(8, 0, "periodic"), (8, 8, "open"), (2, 1, "periodic"),
+                                      (3, 1, "open"), (26, 1, "periodic")])
+def test_sector_edges(dev, L, Lup, bc):
+    """Edge sectors: D = 1 (no flippable bond at all), the two-site chain (its
+    periodic bonds coincide), a chain too short for next-nearest bonds, and
+    the largest L with rank tables at one particle: connectivity, apply and a
+    filter against the oracle."""
+    bonds, Js, Ds = om.j1j2_bonds(L, 1.0, 0.4, 0.9, 1.1, bc) if L >= 4 else om.xxz_bonds(L, 1.0, 0.8, bc)
+    h = np.random.default_rng(L + Lup).uniform(-1, 1, L)
+    from polfed_amd.operators import SpinChainOperator
+    op = SpinChainOperator(L, Lup, bonds, Js, Ds, h, device=dev)
+    H = om.spin_half_hamiltonian(L, bonds, Js, Ds, h, Lup)
+    assert op.D == H.shape[0]
+    cols, vals, counts, diag = op.connectivity(max_per_row=128)
+    assert np.array_equal(diag, H.diagonal())
+    X = np.random.default_rng(1).standard_normal((op.D, 3))
+    assert _rel(dev.to_host(op.apply(dev.to_device(X))), H @ X) < 1e-13
+    g = abs(H).sum(axis=1).max() + 1.0
+    op.set_rescale(-g, g)
+    Ht = H / g
+    c = oc.filter_coefficients(0.1, 30)
+    assert _rel(dev.to_host(op.filter(c, dev.to_device(X))), oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)) < 1e-12
+
+
+@pytest.mark.parametrize("dtype,s,tol", [("f64", 40, 1e-12), ("f32", 70, 5e-4)])
+def test_filter_and_kpm_span_several_column_chunks(dev, dtype, s, tol):
+    """Blocks wider than one internal chunk (32 FP64 / 64 FP32 columns) and
+    more KPM probes than one chunk: every chunk against the oracle."""
+    from polfed_amd.blockops import fill_probes
+    op, H = _chain(dev, *CHAINS[2][1:])
+    g = abs(H).sum(axis=1).max()
+    op.set_rescale(-g, g)
+    Ht = H / g
+    c = oc.filter_coefficients(-0.05, 50)
+    X = np.random.default_rng(5).standard_normal((H.shape[0], s))
+    Y = dev.to_host(op.filter(c, dev.to_device(X), dtype=dtype))
+    assert _rel(Y, oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)) < tol
+    P = fill_probes(dev, H.shape[0], s, seed=9)
+    mu = op.kpm_moments(24, P, dtype=dtype)
+    mu_ref = odos.dos_moments(lambda v: Ht @ v, 24, dev.to_host(P))
+    assert np.max(np.abs(mu - mu_ref)) / H.shape[0] < (1e-12 if dtype == "f64" else 2e-4)

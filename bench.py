@@ -493,11 +493,11 @@ def copy_calibration():
     print(json.dumps({"mode": "copy-calibration", "bytes_per_copy": a.numel() * 4}), flush=True)
 
 
-def solve(op, x0, fp="f64"):
+def solve(op, x0, fp="f64", basistype=None):
     from polfed_amd import DoSConfig, FactorizationConfig, TransformConfig, polfed
     return polfed(op, x0, 100, "middle", produce_report=True,
                   transform=TransformConfig(dtype=fp),
-                  fact=FactorizationConfig(eigentol=1e-10), dos=DoSConfig())
+                  fact=FactorizationConfig(eigentol=1e-10, basistype=basistype), dos=DoSConfig())
 
 
 def cpu_baseline(op, report, x0_host, budget_s=15.0):
@@ -580,6 +580,9 @@ def main():
                          "one polfed call per instance)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="profiling driver: one 256 MiB device copy, five times")
+    ap.add_argument("--hybrid-basis", type=int, default=-1, metavar="DEVICE_VECTORS",
+                    help="also solve with a hybrid basis of this many device columns "
+                         "(the rest in pinned host memory) and report its cost")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "single-GPU build: only --gpus 1 is supported; "
@@ -676,6 +679,28 @@ def main():
             out[f"fp32_filter_{mode}"] = entry
             log(f"{mode}: {em:.2f} s pairs={len(vm)} fp32 iters={lz.fast_iterations}/{lz.numiter} "
                 f"maxres={entry['max_residual_all']}")
+    if args.hybrid_basis >= 0:
+        import numpy as np
+        from polfed_amd import HybridMatrixBasis
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        vh, _, rh = solve(op, x0, basistype=HybridMatrixBasis(max_device_vectors=args.hybrid_basis))
+        torch.cuda.synchronize()
+        eh = time.perf_counter() - t1
+        lz = rh.lanczos
+        n = min(len(vh), len(vals))
+        out["hybrid_basis"] = {
+            "time_s": eh, "slowdown_vs_device_basis": eh / (elapsed / args.steps),
+            "device_vectors": int(lz.basis_device_vectors), "host_vectors": int(lz.basis_host_vectors),
+            "krylovdim": int(lz.krylovdim), "returned_pairs": int(len(vh)),
+            "max_residual": float(rh.eigenresidual),
+            "max_eig_diff_over_width": float(np.max(np.abs(np.sort(vh)[:n] - np.sort(vals)[:n]))
+                                             / (rep.Emax - rep.Emin)) if n else None,
+            "reorth_s": rh.times.get("lanczos.reorthogonalization"),
+            "device_basis_reorth_s": rep.times.get("lanczos.reorthogonalization"),
+        }
+        log(f"hybrid basis: {eh:.2f} s ({lz.basis_device_vectors} device + "
+            f"{lz.basis_host_vectors} host columns)")
     if not args.no_cpu:
         q = dev.to_host(x0)
         out["cpu_baseline"] = cpu_baseline(op, rep, q)

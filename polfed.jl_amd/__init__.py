@@ -6,9 +6,9 @@ in ``lib/libpolfed_hip.so`` (C ABI in ``include/polfed_hip.h``); this package
 is the host-side mirror of the reference's Julia API around it.
 """
 from ._lib import NumericBreakdown, PolfedHIPError  # noqa: F401
-from .configs import (DoSConfig, FactorizationConfig, FullRO, MappingConfig,  # noqa: F401
-                      MulColsParallel, NoParallel, PartialRO, PolfedDefaults, TransformConfig,
-                      TwoLevelParallel)
+from .configs import (DoSConfig, FactorizationConfig, FullRO, HybridMatrixBasis,  # noqa: F401
+                      MappingConfig, MatrixBasis, MulColsParallel, NoParallel, PartialRO,
+                      PolfedDefaults, TransformConfig, TwoLevelParallel)
 
 __version__ = "0.1.0"
 

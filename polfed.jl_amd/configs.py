@@ -125,19 +125,41 @@ class TransformConfig:
     switch_tol: float = 1e-6
 
 
+class MatrixBasis:
+    """Basis storage tag: every Krylov vector in device memory
+    (src/Lanczos/OrthonormalBasis/MatrixBasis.jl; PolfedDefaults.jl:170)."""
+
+
+@dataclass
+class HybridMatrixBasis:
+    """Basis storage tag: device columns first, the rest in pinned host memory
+    (src/Lanczos/OrthonormalBasis/HybridMatrixBasis.jl:8-114).  Pass the class
+    or an instance.  ``max_device_vectors`` None sizes the device part at 90 %
+    of the free device memory, as ``createbasis`` does
+    (OrthonormalBasis.jl:19-59); ``chunk_bytes`` is the size of each of the two
+    device staging buffers through which host columns are streamed."""
+
+    max_device_vectors: Optional[int] = None
+    chunk_bytes: int = 512 << 20
+
+
 @dataclass
 class FactorizationConfig:
-    """Krylov factorization options (FactorizationConfig.jl:49-70)."""
+    """Krylov factorization options (FactorizationConfig.jl:49-70).
+    ``basistype``: ``MatrixBasis`` (default) or ``HybridMatrixBasis``."""
 
     rot: ReOrthTechnique = None
     which: str = PolfedDefaults.which
     tol: float = PolfedDefaults.tol
     eigentol: Optional[float] = PolfedDefaults.eigentol
     overestimate_iters: float = PolfedDefaults.overestimate_iters
+    basistype: object = None
 
     def __post_init__(self):
         if self.rot is None:
             self.rot = FullRO()
+        if self.basistype is None:
+            self.basistype = MatrixBasis
 
 
 @dataclass

@@ -288,7 +288,7 @@ def polfed(op: Operator, x0, howmany: int, target, *, produce_report: bool = Fal
     res = lanczos(dev, f_t, prep.x0, howmany, maxdim=prep.maxdim, which=fact.which, tol=fact.tol,
                   eigentol=fact.eigentol if fact.eigentol is not None else 1e-8, mapvals=op,
                   rot=fact.rot, f_fast=f_fast, fast_mode=fast_mode,
-                  switch_tol=transform.switch_tol)
+                  switch_tol=transform.switch_tol, basistype=fact.basistype)
     torch.cuda.synchronize()
     times["lanczos"] = time.perf_counter() - t0
     times.update({f"lanczos.{k}": v for k, v in res.times.items()})
@@ -371,7 +371,7 @@ def polfed_batch(ops, x0s, howmany: int, target, *, produce_report: bool = False
     gens = [lanczos_coroutine(dev, p.x0, howmany, maxdim=p.maxdim, which=fact.which, tol=fact.tol,
                               eigentol=eigentol, mapvals=op, rot=fact.rot, stage_timers=False,
                               has_fast=transform.dtype != "f64", fast_mode=fast_mode,
-                              switch_tol=transform.switch_tol)
+                              switch_tol=transform.switch_tol, basistype=fact.basistype)
             for op, p in zip(ops, preps)]
     t0 = time.perf_counter()
     pending = {i: next(g) for i, g in enumerate(gens)}

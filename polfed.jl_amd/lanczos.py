@@ -7,7 +7,8 @@ BlockLanczosFactorization.jl / FactorizationSteps/*.jl, the convergence checks
 of Convergence/ConvergenceInfo.jl, CalculatingResiduals.jl, NextChecking.jl,
 getvecs.jl, getvals.jl and the final ordering of LanczosMethod.jl:55-83.
 
-Device work (all FP64): basis storage (one D x maxdim column-major tensor),
+Device work (all FP64): basis storage (one D x maxdim column-major tensor, or
+device columns plus pinned host columns streamed through the device: basis.py),
 QR of the residual block (Cholesky-QR2, Householder on the host only when the
 block is numerically rank deficient), overlaps and three-term updates on the
 FP64 matrix cores, full reorthogonalization (``passes`` CGS sweeps), Ritz
@@ -27,6 +28,7 @@ import torch
 
 from . import blockops as bo
 from ._lib import NumericBreakdown
+from .basis import make_basis
 from .configs import FullRO, PartialRO
 from .device import Device
 
@@ -52,6 +54,8 @@ class LanczosResult:
     fast_iterations: int = 0          # iterations whose filter ran in the fast (FP32) form
     switch_iteration: int = -1        # iteration at which the mixed mode switched to FP32
     fallback: bool = False            # mixed mode went back to FP64 after a failed residual check
+    basis_device_vectors: int = 0     # basis columns held in device memory
+    basis_host_vectors: int = 0       # basis columns held in pinned host memory (hybrid basis)
 
 
 def sortvals(vals: np.ndarray, which: str) -> np.ndarray:
@@ -115,7 +119,7 @@ class _Timer:
 def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | None = None,
             which: str = "SR", tol: float = 1e-14, eigentol: float = 1e-8, mapvals=None,
             rot=None, stage_timers: bool = True, f_fast=None, fast_mode: str = "never",
-            switch_tol: float = 1e-6) -> LanczosResult:
+            switch_tol: float = 1e-6, basistype=None) -> LanczosResult:
     """Block Lanczos for ``howmany`` pairs of the operator ``f``.
 
     ``f(X, Y)`` writes the operator applied to the (s, D) block X into Y.
@@ -130,11 +134,14 @@ def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | Non
     ``switch_tol`` (inexact-Krylov relaxation: late products may be inexact,
     early ones may not), and returns to ``f`` if the final residuals miss
     ``eigentol``.
+
+    ``basistype`` selects the basis storage (``MatrixBasis`` default,
+    ``HybridMatrixBasis``: see basis.py).
     """
     gen = lanczos_coroutine(dev, x0, howmany, maxdim=maxdim, which=which, tol=tol,
                             eigentol=eigentol, mapvals=mapvals, rot=rot,
                             stage_timers=stage_timers, has_fast=f_fast is not None,
-                            fast_mode=fast_mode, switch_tol=switch_tol)
+                            fast_mode=fast_mode, switch_tol=switch_tol, basistype=basistype)
     try:
         X, Y, fast = next(gen)
         while True:
@@ -164,7 +171,7 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
                       maxdim: int | None = None, which: str = "SR", tol: float = 1e-14,
                       eigentol: float = 1e-8, mapvals=None, rot=None, stage_timers: bool = True,
                       has_fast: bool = False, fast_mode: str = "never",
-                      switch_tol: float = 1e-6):
+                      switch_tol: float = 1e-6, basistype=None):
     """The block Lanczos of ``lanczos`` as a generator: it yields every
     operator application it needs as ``(X, Y, fast)`` -- write f(X) (or
     f_fast(X) when ``fast``) into Y, then resume it with ``send(None)`` -- and
@@ -179,7 +186,7 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
     maxiter = math.ceil(maxdim / s)
     maxdim = maxiter * s
     tm = _Timer(stage_timers)
-    basis = torch.empty((maxdim, D), dtype=torch.float64, device=X0.device)
+    basis = make_basis(dev, basistype, maxdim, D, s, X0.device)
     mat = np.zeros((maxdim, maxdim))
     r = torch.empty((s, D), dtype=torch.float64, device=X0.device)
 
@@ -190,13 +197,15 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
         return bo.gram(dev, V, W).cpu().numpy().T
 
     # ---- lanczositer (Factorization.jl:50-67)
-    basis[0:s].copy_(X0)
+    V0 = basis.block(0)
+    V0.copy_(X0)
+    basis.commit(0)
     k = s
     with tm("mapping"):
-        yield basis[0:s], r, False
-    a = sym_upper(overlap(basis[0:s], r))
+        yield V0, r, False
+    a = sym_upper(overlap(V0, r))
     mat[0:s, 0:s] = a
-    bo.update(dev, basis[0:s], _to_dev_matrix(a, r), r, alpha=-1.0, beta=1.0)
+    bo.update(dev, V0, _to_dev_matrix(a, r), r, alpha=-1.0, beta=1.0)
     numiter = 1
     nextcheck = max(howmany // s + 1, 2)
     converged = eigenconverged = 0
@@ -214,12 +223,14 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
     while True:
         # [1] tridiagonalization!: QR of r into the next basis slot
         with tm("tridiagonalization"):
-            slot = basis[k:k + s]
+            basis.prepare(k)
+            slot = basis.block(k)
             slot.copy_(r)
             try:
                 R = bo.block_qr(dev, slot)
             except NumericBreakdown:
                 R = _host_qr(slot)
+            basis.commit(k)
             mat[k:k + s, k - s:k] = R
             mat[k - s:k, k:k + s] = R.T
             k += s
@@ -240,7 +251,7 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
                 if est < switch_tol:
                     use_fast = True
                     switch_it = numiter + 1
-        V_last, V_prev = basis[k - s:k], basis[k - 2 * s:k - s]
+        V_last, V_prev = basis.block(k - s), basis.block(k - 2 * s)
         # [2] mapping!
         with tm("mapping"):
             yield V_last, r, use_fast
@@ -261,7 +272,7 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
                 it = k // s
                 do_ro = do_ro and (it % (2 + rot.skip) == 0 or (it - 1) % (2 + rot.skip) == 0)
             if do_ro:
-                bo.reorth(dev, basis[0:nb], r, passes=rot.passes)
+                basis.reorth(r, nb, rot.passes)
         # [6] getnumberofconvergedvecs!
         numiter += 1
         if nextcheck == numiter or numiter >= maxiter:
@@ -286,8 +297,7 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
                     Phi = phi[:, idxs[:converged]]
                     vecs = torch.empty((converged, D), dtype=torch.float64, device=X0.device)
                     if converged:
-                        bo.update(dev, basis[0:k], _to_dev_matrix(Phi, r), vecs, alpha=1.0,
-                                  beta=0.0)
+                        basis.combine(_to_dev_matrix(Phi, r), vecs, k)
                         vals, res_out = _eigen_residuals(dev, mapvals, vecs)
                     else:
                         vals, res_out = np.zeros(0), np.zeros(0)
@@ -310,6 +320,7 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
         if converged == howmany or numiter >= maxiter:
             break
 
+    basis.close()
     nconv = min(converged, eigenconverged)
     order = np.argsort(vals[:nconv], kind="stable")
     idx = torch.as_tensor(order, device=X0.device, dtype=torch.long)
@@ -317,7 +328,7 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
     return LanczosResult(vals[:nconv][order], vecs_sorted, res_out[:nconv][order], converged,
                          eigenconverged, numiter, maxiter, k, residual, eigenresidual, hist_c,
                          hist_k, hist_r, dict(tm.t), all_vals, all_res, fast_iters, switch_it,
-                         fallback)
+                         fallback, basis.device_vectors, basis.host_vectors)
 
 
 def _eigen_residuals(dev: Device, mapvals, vecs: torch.Tensor):

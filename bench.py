@@ -403,6 +403,86 @@ def xxz_l22(args):
     print(json.dumps(out), flush=True)
 
 
+def xxz_l14(args):
+    """The reference's CPU-runnable case: random-field XXZ L=14, Sz=0 (D=3432),
+    W=5 seed 0, single-vector Lanczos, nev=50, target middle, reference
+    defaults, eigentol 1e-10.  The GPU FP64 solve is timed against the
+    oracle's full CPU solve on the same start vector and KPM probes (the
+    eigenvalues are compared when both pick the same order), and the oracle's C
+    restatement of the reference's CPU filter is timed on one thread, the way
+    the reference runs single-vector filters (ParallelizationClenshaw.jl:55-70)."""
+    import numpy as np
+    import torch
+    from oracle import cpu as ocpu
+    from oracle import models as om
+    from oracle import packing as opk
+    from oracle import polfed as opf
+    from polfed_amd import FactorizationConfig, models, polfed
+    from polfed_amd import spectral as spc
+    from polfed_amd.blockops import fill_probes
+    from polfed_amd.device import Device
+    dev = Device(0)
+    L, nev = 14, 50
+    h = om.resolve_fields(L, 0.0, 5.0, rng=np.random.default_rng(0))
+    op = models.xxz_hamiltonian(L, L // 2, 1.0, 1.0, 5.0, fields=h, device=dev)
+    H = om.xxz_hamiltonian(L, L // 2, 1.0, 1.0, 5.0, fields=h)
+    x0 = np.random.default_rng(100).standard_normal(op.D)
+    x0 /= np.linalg.norm(x0)
+    P = fill_probes(dev, op.D, 300, seed=0x5EED)
+
+    def run():
+        return polfed(op, x0, nev, ("rescaled", 0.0), produce_report=True,
+                      fact=FactorizationConfig(eigentol=1e-10), probes=P)
+    run()  # warm-up: autotuning and first launches
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vals, vecs, rep = run()
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    psi = dev.to_host(vecs)
+    width = rep.Emax - rep.Emin
+    out = {"metric": METRIC, "workload": "XXZ L=14 Sz=0 D=3432 W=5 seed 0, single vector, "
+                                         "nev=50, target middle, eigentol 1e-10",
+           "unit": "s", "higher_is_better": False,
+           "gpu_f64": {"time_s": t_gpu, "order": rep.order, "iterations": rep.iterations,
+                       "returned_pairs": int(len(vals)), "max_residual": rep.eigenresidual,
+                       "max_residual_vs_host_matrix": float(
+                           np.linalg.norm(H @ psi - psi * vals, axis=0).max()) if len(vals) else None,
+                       "times_s": rep.times}}
+    log(f"gpu: {t_gpu:.3f} s K={rep.order} iters={rep.iterations} pairs={len(vals)}")
+    t0 = time.perf_counter()
+    ref = opf.polfed(lambda X: H @ X, x0, nev, ("rescaled", 0.0), eigentol=1e-10,
+                     probes=dev.to_host(P))
+    t_cpu = time.perf_counter() - t0
+    same_k = ref.plan.order == rep.order
+    out["cpu_oracle_full_solve"] = {
+        "time_s": t_cpu, "kind": "port", "order": ref.plan.order,
+        "iterations": ref.lanczos.numiter, "returned_pairs": int(len(ref.vals)),
+        "threads": "1 for the sparse products (scipy CSR), numpy BLAS for the small dense algebra",
+        "max_eig_diff_vs_gpu_over_width": (float(np.max(np.abs(np.sort(vals) - ref.vals)) / width)
+                                           if same_k and len(vals) == len(ref.vals) else None),
+        "speedup_gpu_vs_cpu": t_cpu / t_gpu}
+    log(f"oracle full solve: {t_cpu:.2f} s K={ref.plan.order} pairs={len(ref.vals)}")
+    d, bk = opk.get_diags_and_offdiagonals_by_value(H)
+    dr, bkr = opk.rescaled_packing(d, bk, rep.Emin, rep.Emax)
+    Pc = ocpu.PackedCPU(dr, bkr)
+    c = spc.normalized_coefficients(spc.PolfedDefaults.coefficients, rep.target, rep.order)
+    X1 = x0.reshape(-1, 1).copy()
+    Pc.filter(c, X1, nthreads=1)  # warm-up
+    t0 = time.perf_counter()
+    napps = 5
+    for _ in range(napps):
+        Pc.filter(c, X1, nthreads=1)
+    t_app = (time.perf_counter() - t0) / napps
+    out["cpu_baseline"] = {
+        "value": t_app * rep.iterations, "unit": "s", "cores": 1, "kind": "port",
+        "sample": (f"oracle/cpu_ref.c packed-bucket Clenshaw filter on one thread (the "
+                   f"reference's single-vector CPU filter is single-threaded): {napps} whole "
+                   f"applications of {rep.order + 1} steps, {t_app * 1e3:.2f} ms each, x "
+                   f"{rep.iterations} Lanczos iterations (filter only)")}
+    print(json.dumps(out), flush=True)
+
+
 def j1j2_l24(args):
     """J1-J2 L=24 Sz=0 (D=2704156), J1=1, J2=0.5: KPM with 32 random-phase
     probes and 2048 Chebyshev moments (Jackson), FP32 against FP64 on the same
@@ -571,8 +651,9 @@ def main():
     ap.add_argument("--reorth-only", type=int, default=0, metavar="CALLS",
                     help="profiling driver: time CALLS CGS2 reorthogonalizations")
     ap.add_argument("--workload", default="xxz-l20",
-                    choices=["xxz-l20", "qsun-batch", "xxz-l22", "j1j2-l24"],
-                    help="xxz-l20: the bench line; qsun-batch: 64 Quantum-Sun instances; "
+                    choices=["xxz-l20", "xxz-l14", "qsun-batch", "xxz-l22", "j1j2-l24"],
+                    help="xxz-l20: the bench line; xxz-l14: the CPU-runnable case against the "
+                         "oracle's full CPU solve; qsun-batch: 64 Quantum-Sun instances; "
                          "xxz-l22: FP32/FP64 precision report; j1j2-l24: KPM and filter at L=24")
     ap.add_argument("--instances", type=int, default=64, help="qsun-batch: number of instances")
     ap.add_argument("--modes", default="mixed,f64",
@@ -604,6 +685,8 @@ def main():
         return xxz_l22(args)
     if args.workload == "j1j2-l24":
         return j1j2_l24(args)
+    if args.workload == "xxz-l14":
+        return xxz_l14(args)
     import numpy as np
     import torch
     from polfed_amd.device import Device

@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import dataclasses
 import time
+import warnings
 from dataclasses import dataclass, field
 from typing import Optional
 
@@ -79,6 +80,8 @@ def prepare_initial_state(dev: Device, x0) -> torch.Tensor:
         tol = initial_state_tolerance()
         if abs(nrm - 1.0) <= max(tol, tol * max(nrm, 1.0)):
             return x0
+        warnings.warn(f"POLFED initial vector is not normalized; using a normalized copy "
+                      f"(norm {nrm:.6g}, tolerance {tol:g}).", stacklevel=3)
         return x0 * (1.0 / nrm)
     s, D = x0.shape
     if D == 0:
@@ -93,8 +96,11 @@ def prepare_initial_state(dev: Device, x0) -> torch.Tensor:
         raise ValueError("POLFED initial matrix must contain finite values.")
     tol = initial_state_tolerance()
     I = np.eye(s)
-    if np.linalg.norm(G - I) <= max(tol, tol * max(np.linalg.norm(G), np.linalg.norm(I))):
+    defect = np.linalg.norm(G - I)
+    if defect <= max(tol, tol * max(np.linalg.norm(G), np.linalg.norm(I))):
         return x0
+    warnings.warn(f"POLFED initial matrix is not orthonormal; using an orthonormalized copy "
+                  f"(Gram defect {defect:.6g}, tolerance {tol:g}).", stacklevel=3)
     q = x0.clone()
     try:
         bo.block_qr(dev, q)

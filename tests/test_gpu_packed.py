@@ -209,3 +209,54 @@ def test_packed_rejects_inconsistent_arrays(dev):
                                      i64p(np.ascontiguousarray(starts)), _lib.iptr(fl), n,
                                      ctypes.byref(out))
         assert rc != 0 and not out.value
+
+
+def test_small_polfed_problem_on_the_device(dev):
+    """test/test_polfed.jl:29-61 and its CPU/GPU ordering checks (:166-197):
+    diag(-3, -1.25, -0.2, 0.55, 1.8, 3.2), block 2, three pairs at 0.0, order
+    12, DoS N=8 R=2, tol 1e-11, eigentol 1e-7, no Krylov overestimate.  The
+    six-dimensional space is exhausted after three blocks; whatever pairs come
+    back are eigenpairs of the matrix (residual <= 1e-6 max(1, |E|)), in
+    ascending order, with one column per value."""
+    from polfed_amd import DoSConfig, FactorizationConfig, MappingConfig, TransformConfig, polfed
+    mat = np.diag([-3.0, -1.25, -0.2, 0.55, 1.8, 3.2])
+    q, _ = np.linalg.qr(np.random.default_rng(1234).standard_normal((6, 2)))
+    vals, vecs = polfed(mat, q, 3, 0.0, mapping=MappingConfig(Emin=-3.0, Emax=3.2),
+                        transform=TransformConfig(order=12),
+                        fact=FactorizationConfig(tol=1e-11, eigentol=1e-7, overestimate_iters=1.0),
+                        dos=DoSConfig(N=8, R=2))
+    psi = dev.to_host(vecs).reshape(6, -1)
+    assert psi.shape[1] == len(vals) <= 3
+    assert np.all(np.diff(vals) >= 0)
+    for j, v in enumerate(vals):
+        assert np.linalg.norm(mat @ psi[:, j] - v * psi[:, j]) <= 1e-6 * max(1.0, abs(v))
+        assert np.min(np.abs(np.diag(mat) - v)) <= 1e-6
+
+
+def test_device_initial_state_checks(dev):
+    """_prepare_polfed_initial_state on device blocks (test/test_polfed.jl:138-164):
+    a copy is normalized / orthonormalized with a warning, the caller's array
+    is untouched, and empty, zero or over-wide starts are rejected."""
+    import warnings
+    from polfed_amd.core import initial_state_tolerance, prepare_initial_state
+    assert initial_state_tolerance() == 1e-10
+    v0 = np.array([3.0, 4.0])
+    v = v0.copy()
+    with pytest.warns(UserWarning, match="initial vector is not normalized"):
+        vp = prepare_initial_state(dev, v)
+    assert np.array_equal(v, v0)
+    assert abs(np.linalg.norm(dev.to_host(vp)) - 1.0) < 1e-14
+    x0 = np.array([[1.0, 1.0], [0.0, 1.0], [1.0, 0.0]])
+    x = x0.copy()
+    with pytest.warns(UserWarning, match="initial matrix is not orthonormal"):
+        xp = dev.to_host(prepare_initial_state(dev, x))
+    assert np.array_equal(x, x0)
+    assert np.allclose(xp.T @ xp, np.eye(2), atol=1e-14)
+    # an orthonormal start is used as given, without a warning
+    q = np.linalg.qr(np.random.default_rng(0).standard_normal((5, 2)))[0]
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        assert np.array_equal(dev.to_host(prepare_initial_state(dev, q)), q)
+    for bad in (np.zeros(3), np.ones((2, 3)), np.zeros(0)):
+        with pytest.raises(ValueError):
+            prepare_initial_state(dev, bad)

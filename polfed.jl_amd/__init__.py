@@ -19,6 +19,18 @@ def polfed(*args, **kwargs):
     return _polfed(*args, **kwargs)
 
 
+def lanczos_extrema(*args, **kwargs):
+    """See ``polfed_amd.core.lanczos_extrema``."""
+    from .core import lanczos_extrema as _lanczos_extrema
+    return _lanczos_extrema(*args, **kwargs)
+
+
+def display_report(*args, **kwargs):
+    """See ``polfed_amd.core.display_report``."""
+    from .core import display_report as _display_report
+    return _display_report(*args, **kwargs)
+
+
 def polfed_batch(*args, **kwargs):
     """See ``polfed_amd.core.polfed_batch``."""
     from .core import polfed_batch as _polfed_batch

@@ -110,13 +110,14 @@ def prepare_initial_state(dev: Device, x0) -> torch.Tensor:
     return q
 
 
-def _extremal(dev: Device, op: Operator, v0: torch.Tensor, which: str, tol: float = 1e-14,
-              eigentol: float = 1e-8) -> float:
+def _extremal(dev: Device, op, v0: torch.Tensor, which: str, tol: float = 1e-14,
+              eigentol: float = 1e-8, maxdim: int = 1000) -> float:
     """``lanczos(f!, v0, 1; which, maxdim=1000)`` probe of build_mapping_plan
     (Plan.jl:136-168) with the unscaled operator (tolerances of
-    optimize_spectral_transform's probe: optimization.jl:40-52)."""
-    f = lambda X, Y: op.apply(X, Y)
-    res = lanczos(dev, f, v0, 1, maxdim=1000, which=which, tol=tol, eigentol=eigentol,
+    optimize_spectral_transform's probe: optimization.jl:40-52).  ``op`` is an
+    Operator or a callable f(X, Y) on device blocks."""
+    f = (lambda X, Y: op.apply(X, Y)) if isinstance(op, Operator) else op
+    res = lanczos(dev, f, v0, 1, maxdim=maxdim, which=which, tol=tol, eigentol=eigentol,
                   mapvals=op, stage_timers=False)
     if res.vals.size == 0:
         raise RuntimeError(f"Failed to estimate {which} extremal eigenvalue with Lanczos. "
@@ -245,6 +246,74 @@ def _prepare(op: Operator, x0, howmany: int, target, mapping: MappingConfig,
     if transform.dtype not in ("f64", "f32", "mixed"):
         raise ValueError(f"TransformConfig.dtype must be 'f64', 'f32' or 'mixed', got {transform.dtype!r}")
     return _Prepared(x0, s, Emin, Emax, plan, coeffs, moments, maxdim, times)
+
+
+def lanczos_extrema(op, x0=None, *, maxdim: int = 1000, tol: float = 1e-14,
+                    eigentol: float = 1e-8, seed: Optional[int] = None,
+                    device: Optional[Device] = None):
+    """``lanczos_extrema`` (src/Lanczos/Lanczos.jl:146-212): ``(Emin, Emax)`` from
+    two single-vector Lanczos probes (which "SR", then "LR") on the device.
+
+    ``op`` is a GPU operator, a square real symmetric matrix (packed like the
+    matrix input of ``polfed``) or a callable ``f(X, Y)`` writing the operator
+    applied to the device block X into Y.  Both probes start from a normalized
+    copy of ``x0`` (host array or device vector; the caller's array is not
+    changed); without ``x0`` an operator or matrix gets a random normal start
+    (``seed``).  A zero start, a non-square matrix or a start of the wrong
+    length raise ``ValueError``."""
+    dev = device or (op.device if isinstance(op, Operator) else default_device())
+    if isinstance(op, Operator):
+        D = op.D
+    elif callable(op):
+        if x0 is None:
+            raise ValueError("lanczos_extrema with an operator callable needs x0.")
+        D = int(x0.shape[0])
+    else:
+        shape = getattr(op, "shape", None)
+        if shape is None or len(shape) != 2 or shape[0] != shape[1]:
+            raise ValueError("lanczos_extrema requires a square matrix.")
+        D = int(shape[0])
+    if x0 is None:
+        x0 = np.random.default_rng(seed).standard_normal(D)
+    if isinstance(x0, torch.Tensor):
+        v = x0.detach().to(dev.torch_device, dtype=torch.float64).reshape(-1).clone()
+    else:
+        v = dev.to_device(np.asarray(x0, dtype=np.float64).reshape(-1))
+    if v.numel() != D:
+        raise ValueError("x0 length must match the matrix dimension.")
+    nrm = float(np.sqrt(bo.coldots(dev, v, v)[0]))
+    if nrm == 0 or not np.isfinite(nrm):
+        raise ValueError("lanczos_extrema requires a nonzero, finite starting vector.")
+    v = v * (1.0 / nrm)
+    if not isinstance(op, Operator) and not callable(op):
+        op = PackedOperator.from_matrix(op, device=dev)
+    return (_extremal(dev, op, v, "SR", tol, eigentol, maxdim),
+            _extremal(dev, op, v, "LR", tol, eigentol, maxdim))
+
+
+def display_report(report: "Report", file=None) -> str:
+    """Text summary of a ``polfed`` report (the reference's display_report,
+    src/Polfed/Structs/Report/*.jl): spectral transform, factorization and
+    stage times.  Printed to ``file`` (default stdout) and returned."""
+    r = report
+    lines = [
+        "POLFED report",
+        f"  spectrum      Emin = {r.Emin:.12g}, Emax = {r.Emax:.12g}",
+        f"  target        {r.target:.12g} (rescaled), interval [{r.left:.6g}, {r.right:.6g}]",
+        f"  filter        order {r.order}, {r.filter_dtype}",
+        f"  factorization block {r.blocksize}, maxdim {r.maxdim}, iterations {r.iterations}",
+        f"  converged     {r.converged} (Lanczos), {r.eigenconverged} (eigen-residual)",
+        f"  residuals     Lanczos {r.residual:.3e}, eigen {r.eigenresidual:.3e}",
+        f"  operator applications {r.num_vecmuls}",
+    ]
+    if r.lanczos is not None and r.lanczos.basis_host_vectors:
+        lines.append(f"  basis         {r.lanczos.basis_device_vectors} device + "
+                     f"{r.lanczos.basis_host_vectors} host columns")
+    for k, t in r.times.items():
+        lines.append(f"  time {k:<28s} {t:10.4f} s")
+    text = "\n".join(lines)
+    print(text, file=file)
+    return text
 
 
 def _report(prep: _Prepared, res: LanczosResult, howmany: int, dtype: str, times: dict) -> Report:

@@ -85,6 +85,13 @@ def _to_dev_matrix(M: np.ndarray, like: torch.Tensor) -> torch.Tensor:
     return torch.from_numpy(np.ascontiguousarray(M.T)).to(like.device)
 
 
+def _exhausted(R: np.ndarray, tmax: float) -> bool:
+    """True when the QR factor of the residual block is rounding noise against
+    the projected matrix (||R|| <= 1000 eps max|T|, ``tmax`` = max|T| so far): the
+    Krylov space is invariant, as when it fills a space of dimension <= maxdim."""
+    return float(np.linalg.norm(R)) <= 1000 * np.finfo(np.float64).eps * max(tmax, 1e-300)
+
+
 def _host_qr(block: torch.Tensor) -> np.ndarray:
     """Householder QR on the host for a rank-deficient block (in place on the
     device tensor); the reference's qr(fact.r) (BlockLanczosFactorization.jl:123)."""
@@ -205,6 +212,7 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
         yield V0, r, False
     a = sym_upper(overlap(V0, r))
     mat[0:s, 0:s] = a
+    tmax = float(np.max(np.abs(a)))  # max|mat| so far
     bo.update(dev, V0, _to_dev_matrix(a, r), r, alpha=-1.0, beta=1.0)
     numiter = 1
     nextcheck = max(howmany // s + 1, 2)
@@ -214,6 +222,43 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
     vals = np.zeros(0)
     vecs = torch.empty((0, D), dtype=torch.float64, device=X0.device)
     res_out = np.zeros(0)
+
+    def check(k, bnorm, final):
+        """Residual estimates ||bnorm phi_last|| of the wanted Ritz pairs of
+        mat[:k, :k] (counted while the running maximum stays below tol), and the
+        Ritz vectors with their eigen-residuals once all converged or final."""
+        nonlocal converged, eigenconverged, residual, eigenresidual, vals, vecs, res_out
+        nonlocal all_vals, all_res
+        lam, phi = _proj_eigh(mat[:k, :k], s)
+        idxs = sortvals(lam, which)
+        converged = 0
+        maxres = 0.0
+        for i in range(min(howmany, k)):
+            rr = float(np.linalg.norm(bnorm @ phi[k - s:k, idxs[i]]))
+            if rr > maxres:
+                maxres = rr
+            if maxres < tol:
+                converged += 1
+        residual = maxres
+        hist_c.append(converged)
+        hist_k.append(k)
+        hist_r.append(maxres)
+        if converged == howmany or final:
+            Phi = phi[:, idxs[:converged]]
+            vecs = torch.empty((converged, D), dtype=torch.float64, device=X0.device)
+            if converged:
+                basis.combine(_to_dev_matrix(Phi, r), vecs, k)
+                vals, res_out = _eigen_residuals(dev, mapvals, vecs)
+            else:
+                vals, res_out = np.zeros(0), np.zeros(0)
+            eigenconverged = 0
+            for rv in res_out:
+                if rv < eigentol:
+                    eigenconverged += 1
+                else:
+                    break
+            eigenresidual = float(res_out.max()) if res_out.size else 0.0
+            all_vals, all_res = vals.copy(), res_out.copy()
 
     use_fast = fast_mode == "always" and has_fast
     fast_iters = 0
@@ -230,10 +275,21 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
                 R = bo.block_qr(dev, slot)
             except NumericBreakdown:
                 R = _host_qr(slot)
-            basis.commit(k)
-            mat[k:k + s, k - s:k] = R
-            mat[k - s:k, k:k + s] = R.T
-            k += s
+            invariant = _exhausted(R, tmax)
+            if not invariant:
+                tmax = max(tmax, float(np.max(np.abs(R))))
+                basis.commit(k)
+                mat[k:k + s, k - s:k] = R
+                mat[k - s:k, k:k + s] = R.T
+                k += s
+        if invariant:
+            # the residual block vanished: the basis spans an invariant
+            # subspace and mat[:k, :k] holds its exact eigenpairs (residual
+            # estimate zero; the eigen-residuals still decide); the next block
+            # would be rounding noise, so the run ends here
+            with tm("convergence"):
+                check(k, np.zeros_like(R), True)
+            break
         if fast_mode == "mixed" and has_fast and not use_fast and not fallback \
                 and k - s >= howmany:
             with tm("switch_test"):
@@ -261,6 +317,7 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
         with tm("orthogonalization"):
             a = sym_upper(overlap(V_last, r))
             mat[k - s:k, k - s:k] = a
+            tmax = max(tmax, float(np.max(np.abs(a))))
             b = mat[k - s:k, k - 2 * s:k - s]
             bo.update(dev, V_last, _to_dev_matrix(a, r), r, alpha=-1.0, beta=1.0)
             bo.update(dev, V_prev, _to_dev_matrix(b.T, r), r, alpha=-1.0, beta=1.0)
@@ -277,38 +334,7 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
         numiter += 1
         if nextcheck == numiter or numiter >= maxiter:
             with tm("convergence"):
-                T = mat[:k, :k]
-                lam, phi = _proj_eigh(T, s)
-                idxs = sortvals(lam, which)
-                bnorm = mat[k - s:k, k - 2 * s:k - s]
-                converged = 0
-                maxres = 0.0
-                for i in range(howmany):
-                    rr = float(np.linalg.norm(bnorm @ phi[k - s:k, idxs[i]]))
-                    if rr > maxres:
-                        maxres = rr
-                    if maxres < tol:
-                        converged += 1
-                residual = maxres
-                hist_c.append(converged)
-                hist_k.append(k)
-                hist_r.append(maxres)
-                if converged == howmany or numiter >= maxiter:
-                    Phi = phi[:, idxs[:converged]]
-                    vecs = torch.empty((converged, D), dtype=torch.float64, device=X0.device)
-                    if converged:
-                        basis.combine(_to_dev_matrix(Phi, r), vecs, k)
-                        vals, res_out = _eigen_residuals(dev, mapvals, vecs)
-                    else:
-                        vals, res_out = np.zeros(0), np.zeros(0)
-                    eigenconverged = 0
-                    for rv in res_out:
-                        if rv < eigentol:
-                            eigenconverged += 1
-                        else:
-                            break
-                    eigenresidual = float(res_out.max()) if res_out.size else 0.0
-                    all_vals, all_res = vals.copy(), res_out.copy()
+                check(k, mat[k - s:k, k - 2 * s:k - s], numiter >= maxiter)
                 nextcheck = _next_check(numiter, residual)
         if converged == howmany and fast_mode == "mixed" and use_fast \
                 and eigenconverged < converged and numiter < maxiter:

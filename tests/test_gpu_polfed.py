@@ -149,3 +149,34 @@ def test_polfed_mixed_precision_meets_residual_tolerance(dev):
     # operator coefficients perturb H~ by ~eps32, so agreement is ~1e-8 x width
     assert max(np.min(np.abs(allv - v)) for v in v64) <= 1e-6 * width
     assert lz.all_residuals.max() < 1e-3
+
+
+def test_lanczos_extrema_and_report(dev):
+    """lanczos_extrema (src/Lanczos/Lanczos.jl:146-212; test/test_polfed.jl:94-107)
+    on a matrix, a GPU operator and a callable: the extremal eigenvalues, the
+    caller's start untouched, and the argument errors; display_report on a
+    run's report."""
+    from polfed_amd import DoSConfig, MappingConfig, TransformConfig, display_report, lanczos_extrema, polfed
+    from polfed_amd.models import xxz_hamiltonian
+    mat = np.diag([-3.0, -1.25, -0.2, 0.55, 1.8, 3.2])
+    x0 = np.ones(6)
+    Emin, Emax = lanczos_extrema(mat, x0)
+    assert abs(Emin + 3.0) < 1e-10 and abs(Emax - 3.2) < 1e-10
+    assert np.array_equal(x0, np.ones(6))
+    L = 12
+    h = om.resolve_fields(L, 0.0, 2.0, rng=np.random.default_rng(3))
+    op = xxz_hamiltonian(L, L // 2, 1.0, 1.0, 2.0, fields=h, device=dev)
+    ev = np.linalg.eigvalsh(om.xxz_hamiltonian(L, L // 2, 1.0, 1.0, 2.0, fields=h).toarray())
+    width = ev[-1] - ev[0]
+    for e in (lanczos_extrema(op, seed=1),
+              lanczos_extrema(lambda X, Y: op.apply(X, Y), np.random.default_rng(2).standard_normal(op.D))):
+        assert abs(e[0] - ev[0]) < 1e-9 * width and abs(e[1] - ev[-1]) < 1e-9 * width
+    for bad in [(mat, np.zeros(6)), (np.ones((2, 3)), np.ones(2)), (mat, np.ones(5))]:
+        with pytest.raises(ValueError):
+            lanczos_extrema(*bad)
+    q = np.linalg.qr(np.random.default_rng(4).standard_normal((op.D, 2)))[0]
+    _, _, rep = polfed(op, q, 4, "middle", produce_report=True,
+                       mapping=MappingConfig(Emin=ev[0], Emax=ev[-1]),
+                       transform=TransformConfig(order=60), dos=DoSConfig(N=32, R=8))
+    text = display_report(rep)
+    assert "order 60" in text and "iterations" in text

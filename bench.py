@@ -314,7 +314,7 @@ def qsun_batch(args):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res, brep = polfed_batch(ops, x0s, 32, "middle", produce_report=True, fact=fact,
-                                 transform=TransformConfig(dtype=mode), progress=log)
+                                 transform=TransformConfig(dtype=mode, **_switch(args)), progress=log)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         results[mode] = res
@@ -377,7 +377,7 @@ def xxz_l22(args):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         vals, vecs, rep = polfed(op, x0, 200, "middle", produce_report=True,
-                                 transform=TransformConfig(dtype=mode),
+                                 transform=TransformConfig(dtype=mode, **_switch(args)),
                                  fact=FactorizationConfig(eigentol=1e-10))
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
@@ -573,10 +573,16 @@ def copy_calibration():
     print(json.dumps({"mode": "copy-calibration", "bytes_per_copy": a.numel() * 4}), flush=True)
 
 
-def solve(op, x0, fp="f64", basistype=None):
+def _switch(args):
+    """TransformConfig keywords for a --switch-tol override."""
+    return {} if getattr(args, "switch_tol", None) is None else {"switch_tol": args.switch_tol}
+
+
+def solve(op, x0, fp="f64", basistype=None, switch_tol=None):
     from polfed_amd import DoSConfig, FactorizationConfig, TransformConfig, polfed
+    tc = TransformConfig(dtype=fp) if switch_tol is None else TransformConfig(dtype=fp, switch_tol=switch_tol)
     return polfed(op, x0, 100, "middle", produce_report=True,
-                  transform=TransformConfig(dtype=fp),
+                  transform=tc,
                   fact=FactorizationConfig(eigentol=1e-10, basistype=basistype), dos=DoSConfig())
 
 
@@ -661,6 +667,11 @@ def main():
                          "one polfed call per instance)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="profiling driver: one 256 MiB device copy, five times")
+    ap.add_argument("--switch-tol", type=float, default=None,
+                    help="mixed-precision solves: residual estimate at which the filter "
+                         "switches to FP32 (default: TransformConfig's)")
+    ap.add_argument("--modes-l20", default="mixed,f32",
+                    help="FP32-filter solves reported beside the FP64 bench solve")
     ap.add_argument("--hybrid-basis", type=int, default=-1, metavar="DEVICE_VECTORS",
                     help="also solve with a hybrid basis of this many device columns "
                          "(the rest in pinned host memory) and report its cost")
@@ -742,10 +753,10 @@ def main():
         import numpy as np
         width = rep.Emax - rep.Emin
         ref_vals = np.sort(vals)
-        for mode in ("mixed", "f32"):
+        for mode in args.modes_l20.split(","):
             torch.cuda.synchronize()
             t1 = time.perf_counter()
-            vm, _, rm = solve(op, x0, mode)
+            vm, _, rm = solve(op, x0, mode, switch_tol=args.switch_tol)
             torch.cuda.synchronize()
             em = time.perf_counter() - t1
             lz = rm.lanczos

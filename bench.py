@@ -148,6 +148,8 @@ def filter_only(args):
     from polfed_amd import models
     from polfed_amd.device import Device
     dev = Device(0)
+    if args.workload == "qsun-batch":
+        return batch_filter_only(args, dev)
     if args.workload == "j1j2-l24":
         op = models.j1j2_hamiltonian(24, 12, 1.0, 0.5, 1.0, 1.0, 0.0, device=dev)
         op.set_rescale(-9.0, 9.0)
@@ -182,6 +184,44 @@ def filter_only(args):
                       "layout": args.layout or "autotuned", "dtype": dt,
                       "D": op.D, "s": s, "steps": reps * (K + 1), "us_per_step": us,
                       "algorithmic_GBps": 4 * s * op.D * esz / (us * 1e-6) / 1e9}), flush=True)
+
+
+def batch_filter_only(args, dev):
+    """Profiling driver for the multi-instance filter: the 64 Quantum-Sun
+    instances of the qsun-batch workload, each rescaled by the fixed bounds
+    [-40, 40] (wider than every instance's spectrum) with a 401-coefficient
+    filter, one column each; the autotuned kernel (``--dtype``), then about
+    ``--filter-only`` batched steps in stream mode (one kernel per step, so a
+    profiler sees every step)."""
+    import numpy as np
+    import torch
+    from polfed_amd.batch import QSunBatch
+    ops, x0s = qsun_batch_instances(dev, args.instances)
+    for op in ops:
+        op.set_rescale(-40.0, 40.0)
+    K = 400
+    b = QSunBatch(ops, 1)
+    b.set_filter([np.cos(np.arange(K + 1) * 0.3) * 0.01 for _ in ops])
+    dt = args.dtype
+    tuned_us = b.autotune(dt)
+    b.set_tuning(graph=0)
+    X = [dev.to_device(x) for x in x0s]
+    Y = [torch.empty_like(x) for x in X]
+    reqs = [(x, y, dt) for x, y in zip(X, Y)]
+    b.filter(reqs)
+    torch.cuda.synchronize()
+    reps = max(1, args.filter_only // (K + 1))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.filter(reqs)
+    torch.cuda.synchronize()
+    us = (time.perf_counter() - t0) / (reps * (K + 1)) * 1e6
+    rows = sum(op.D for op in ops)
+    esz = 8 if dt == "f64" else 4
+    print(json.dumps({"mode": "batch-filter-only", "dtype": dt, "instances": len(ops),
+                      "rows": rows, "autotune_us_per_step": tuned_us,
+                      "stream_us_per_step": us, "steps": reps * (K + 1),
+                      "algorithmic_GBps": 4 * rows * esz / (us * 1e-6) / 1e9}), flush=True)
 
 
 def packed_compare(args):

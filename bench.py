@@ -139,8 +139,10 @@ def measured_traffic(kernel):
 def filter_only(args):
     """Profiling driver: `--filter-only` Clenshaw steps of a fixed filter on the
     bench workload's operator and start block (XXZ L=20, FP64 block 8, order
-    200), or with ``--workload j1j2-l24`` on the J1-J2 L=24 operator with a
-    random 32-column block (order 20); the autotuned step kernel unless
+    200), with ``--workload xxz-l22`` on the L=22 operator and 16-column start
+    block of that workload (order 100), or with ``--workload j1j2-l24`` on the
+    J1-J2 L=24 operator with a random 32-column block (order 20); the
+    autotuned step kernel unless
     ``--layout`` fixes it.  Prints one JSON line with the average step time
     (HIP events)."""
     import numpy as np
@@ -155,6 +157,12 @@ def filter_only(args):
         op.set_rescale(-9.0, 9.0)
         s, K = 32, 20
         x0 = dev.to_device(np.random.default_rng(0).standard_normal((op.D, s)))
+    elif args.workload == "xxz-l22":
+        op = models.xxz_hamiltonian(22, 11, 1.0, 1.0, 2.0, rng=np.random.default_rng(2),
+                                    device=dev)
+        op.set_rescale(-18.0, 14.0)
+        s, K = 16, 100
+        x0 = dev.to_device(np.linalg.qr(np.random.default_rng(102).standard_normal((op.D, s)))[0])
     else:
         op, x0 = build_workload(dev)
         op.set_rescale(-17.2, 13.5)

@@ -71,7 +71,9 @@ int pf_destroy(pf_handle* h);
  * and j1j2_hamiltonian (:189-220).  Sites are 1-based like the reference's bond
  * tuples (site k <-> bit k-1); `fields` has L entries (host arrays).
  * Basis order = ascending integers (combinadic rank), bit-exact with the
- * reference's Dict index.  Requires 2 <= L <= 32, nbonds <= 128. */
+ * reference's Dict index.  Requires 2 <= L <= 32, 0 <= Lup <= 18 (the
+ * binomial table's width), nbonds <= 128 and C(L, Lup) < 2^32; any number of
+ * distinct couplings (one value per bond). */
 int pf_op_create_spin_chain(pf_handle* h, int32_t L, int32_t Lup, int32_t nbonds,
                             const int32_t* site_i, const int32_t* site_j,
                             const double* J, const double* Delta, const double* fields,
@@ -115,13 +117,27 @@ int pf_op_set_rescale(pf_op* op, double Emin, double Emax);
 
 /* Export the connectivity the kernels compute on the device, for rows
  * [row0, row0+nrows): for each row, up to max_per_row off-diagonal (column,
- * value) pairs in bond order and the diagonal entry (unscaled H).  Device
+ * value) pairs in the kernels' bond order and the diagonal entry (unscaled H).
+ * The columns come from the step kernels' own rank function (the two rank
+ * tables when L <= 26, else the O(1) combinadic update) and the diagonal from
+ * the table they stream (Quantum Sun: the split Dlo + Dhi table).  Device
  * output buffers: cols/vals are nrows x max_per_row row-major, counts nrows,
  * diag nrows.  This is the parity surface for the integer rank/connectivity
- * work (reference: rows/cols/vals of SpinHalfChains.jl:284-311). */
+ * work (reference: rows/cols/vals of SpinHalfChains.jl:284-311,
+ * QuantumSun.jl:1-102). */
 int pf_op_connectivity(pf_op* op, int64_t row0, int64_t nrows, int32_t max_per_row,
                        int64_t* cols, double* vals, int32_t* counts, double* diag,
                        pf_stream stream);
+
+/* pf_op_connectivity with a row order: 0 = the reference order (what
+ * pf_apply, the Clenshaw hooks and the drivers without the partition order
+ * apply), 1 = the spin-chain XCD partition order the filter and KPM drivers
+ * may run (its rank tables, states and diagonal; rows and columns are mapped
+ * back to reference ranks, so the output is comparable row for row).  Order 1
+ * needs a spin chain with rank tables.  Same reference rows as above. */
+int pf_op_connectivity_order(pf_op* op, int32_t order, int64_t row0, int64_t nrows,
+                             int32_t max_per_row, int64_t* cols, double* vals, int32_t* counts,
+                             double* diag, pf_stream stream);
 
 /* --------------------------------------------------------------- mappings */
 

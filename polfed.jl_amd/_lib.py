@@ -87,6 +87,8 @@ SIGNATURES = {
     "pf_op_dim": (_i64, [_vp]),
     "pf_op_set_rescale": (_c.c_int, [_vp, _dbl, _dbl]),
     "pf_op_connectivity": (_c.c_int, [_vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "pf_op_connectivity_order": (_c.c_int, [_vp, _i32, _i64, _i64, _i32, _vp, _vp, _vp, _vp,
+                                            _vp]),
     "pf_apply": (_c.c_int, [_vp, _i32, _i32, _vp, _i64, _vp, _i64, _i32, _vp]),
     "pf_clenshaw_recurrence": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _dbl, _dbl, _dbl, _vp, _i64,
                                           _i32, _vp]),

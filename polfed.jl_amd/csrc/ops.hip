@@ -541,37 +541,61 @@ __global__ void chain_diag_kernel(const uint32_t* __restrict__ states, int64_t D
   diag[r] = d;
 }
 
-// Connectivity export (parity surface): the bonds of each value class in turn.
-__global__ void chain_conn_kernel(const uint32_t* __restrict__ states,
-                                  const uint32_t* __restrict__ binom, const double* __restrict__ diag,
-                                  int64_t row0, int64_t nrows, int nb, const int8_t* __restrict__ bi,
-                                  const int8_t* __restrict__ bj, const double* __restrict__ halfj,
-                                  int maxper, int64_t* __restrict__ cols, double* __restrict__ vals,
-                                  int32_t* __restrict__ counts, double* __restrict__ dout) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrows) return;
-  const int64_t row = row0 + t;
-  const uint32_t st = states[row];
+// Connectivity export (parity surface).  Each row's neighbours come from the
+// same rank function the step kernels call (one_rank<TBL>: the TA/TB tables
+// when the operator has them, else the O(1) update), over the kernels' bond
+// list (value-grouped, without the zero padding), and the diagonal from the
+// table the step kernels stream.  Reference order: row = rank in the ascending basis.  In
+// the XCD partition order (kRowPart drivers) the kernel runs over partition
+// rows q with the partition tables and writes reference rank pperm[q]'s slot,
+// its neighbours mapped back through pperm: that is the integer path those
+// drivers apply.
+template <bool TBL>
+__global__ void chain_conn_kernel(const ChainParams P, const uint32_t* __restrict__ states,
+                                  const uint32_t* __restrict__ binom, const void* __restrict__ rtab,
+                                  const double* __restrict__ diag,
+                                  const uint32_t* __restrict__ perm, int64_t row0, int64_t nrows,
+                                  int64_t nscan, int maxper, int64_t* __restrict__ cols,
+                                  double* __restrict__ vals, int32_t* __restrict__ counts,
+                                  double* __restrict__ dout) {
+  __shared__ uint32_t sM[kMaxBonds];
+  for (int i = threadIdx.x; i < P.nb; i += blockDim.x) sM[i] = (1u << P.lo[i]) | (1u << P.hi[i]);
+  __syncthreads();
+  RankCtx R{};
+  R.M = sM;
+  if constexpr (TBL) {
+    R.TA = reinterpret_cast<const uint16_t*>(rtab);
+    R.TB = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(rtab) +
+                                             (2u << P.tsplit));
+    R.maskA = (1u << P.tsplit) - 1u;
+    R.m = P.tsplit;
+  } else {
+    R.C = binom;
+  }
+  const int64_t q = (perm ? 0 : row0) + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (perm ? nscan : row0 + nrows)) return;
+  const int64_t ref = perm ? (int64_t)perm[q] : q;
+  if (ref < row0 || ref >= row0 + nrows) return;
+  const int64_t t = ref - row0;
+  const uint32_t st = states[q];
   int cnt = 0;
-  for (int b = 0; b < nb; ++b) {
-    const int i = bi[b], j = bj[b];
-    const int lo = i < j ? i : j, hi = i < j ? j : i;
-    if (((st >> lo) ^ (st >> hi)) & 1u) {
-      if (cnt < maxper) {
-        cols[t * maxper + cnt] = (int64_t)neighbor_rank(st, (uint32_t)row, lo, hi, binom);
-        vals[t * maxper + cnt] = halfj[b];
-      }
-      ++cnt;
+  for (int b = 0; b < P.nb; ++b) {
+    const uint32_t nr = one_rank<TBL>(R, b, st, (uint32_t)q);
+    if (nr == (uint32_t)q) continue;  // parallel spins: no matrix element
+    if (cnt < maxper) {
+      cols[t * maxper + cnt] = perm ? (int64_t)perm[nr] : (int64_t)nr;
+      vals[t * maxper + cnt] = P.bval[b];
     }
+    ++cnt;
   }
   counts[t] = cnt;
-  dout[t] = diag[row];
+  dout[t] = diag[q];
 }
 
-__global__ void qsun_conn_kernel(const QsunParams P, int64_t row0, int64_t nrows, int maxper,
+__global__ void qsun_conn_kernel(const QsunParams P, const double* __restrict__ qdiag, int qlb,
+                                 int64_t row0, int64_t nrows, int maxper,
                                  int64_t* __restrict__ cols, double* __restrict__ vals,
                                  int32_t* __restrict__ counts, double* __restrict__ dout) {
-#pragma clang fp contract(off)
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nrows) return;
   const uint32_t r = (uint32_t)(row0 + t);
@@ -594,14 +618,9 @@ __global__ void qsun_conn_kernel(const QsunParams P, int64_t row0, int64_t nrows
     }
     ++cnt;
   }
-  double dd = 0.0;
-  for (int j = 0; j < P.Lloc; ++j) {
-    const double tz = P.h[j] * (((r >> (P.Lg + j)) & 1u) ? 0.5 : -0.5);
-    dd = dd + tz;
-  }
-  dd = dd + P.G[a * dg + a];
   counts[t] = cnt;
-  dout[t] = dd;
+  // the split diagonal the step kernels read: Dlo[r mod 2^lb] + Dhi[r >> lb]
+  dout[t] = qdiag[r & ((1u << qlb) - 1u)] + qdiag[((size_t)1 << qlb) + (r >> qlb)];
 }
 
 // KPM / probe helpers ---------------------------------------------------
@@ -808,21 +827,15 @@ extern "C" int pf_op_create_spin_chain(pf_handle* h, int32_t L, int32_t Lup, int
         classes.push_back(halfj[b]);
     }
     std::sort(classes.begin(), classes.end());
-    PF_REQUIRE((int)classes.size() <= kMaxClasses, "spin chain: at most 8 distinct couplings");
-    P.ncls = (int)classes.size();
     int nb = 0;
-    for (int c = 0; c < P.ncls; ++c) {
-      P.cls_begin[c] = nb;
-      P.cval[c] = classes[c];
+    for (const double cv : classes)
       for (int b = 0; b < nbonds; ++b)
-        if (halfj[b] == classes[c]) {
+        if (halfj[b] == cv) {
           P.lo[nb] = std::min(oi[b], oj[b]);
           P.hi[nb] = std::max(oi[b], oj[b]);
-          P.bval[nb] = classes[c];
+          P.bval[nb] = cv;
           ++nb;
         }
-    }
-    P.cls_begin[P.ncls] = nb;
     P.nb = nb;  // bonds with a nonzero off-diagonal value, in class order
 
     // basis (ascending, Gosper's hack) and binomial table
@@ -1059,55 +1072,54 @@ extern "C" int pf_op_set_rescale(pf_op* op, double Emin, double Emax) {
   });
 }
 
-extern "C" int pf_op_connectivity(pf_op* op, int64_t row0, int64_t nrows, int32_t max_per_row,
-                                  int64_t* cols, double* vals, int32_t* counts, double* diag,
-                                  pf_stream stream) {
+extern "C" int pf_op_connectivity_order(pf_op* op, int32_t order, int64_t row0, int64_t nrows,
+                                        int32_t max_per_row, int64_t* cols, double* vals,
+                                        int32_t* counts, double* diag, pf_stream stream) {
   return guarded([&] {
     PF_REQUIRE(op, "null operator");
     PF_REQUIRE(row0 >= 0 && nrows >= 0 && row0 + nrows <= op->D, "connectivity: rows out of range");
     PF_REQUIRE(max_per_row >= 1 && cols && vals && counts && diag, "connectivity: bad outputs");
+    PF_REQUIRE(order == 0 || order == 1, "connectivity: order must be 0 (reference) or 1 (partition)");
+    PF_REQUIRE(order == 0 || (op->kind == kChain && op->pstates),
+               "connectivity: this operator has no partition row order");
     if (nrows == 0) return;
     hipStream_t st = S(stream);
-    const unsigned nb = (unsigned)((nrows + 255) / 256);
     if (op->kind == kChain) {
       const ChainParams& P = op->chain;
-      // the export walks the class-sorted bond list the step kernels use
-      std::vector<int8_t> bi(P.nb), bj(P.nb);
-      std::vector<double> hv(P.nb);
-      for (int c = 0; c < P.ncls; ++c)
-        for (int b = P.cls_begin[c]; b < P.cls_begin[c + 1]; ++b) {
-          bi[b] = P.lo[b];
-          bj[b] = P.hi[b];
-          hv[b] = P.cval[c];
-        }
-      int8_t *dbi, *dbj;
-      double* dh;
-      const int nbm = std::max(P.nb, 1);
-      PF_HIP(hipMalloc(&dbi, nbm));
-      PF_HIP(hipMalloc(&dbj, nbm));
-      PF_HIP(hipMalloc(&dh, nbm * sizeof(double)));
-      if (P.nb) {
-        PF_HIP(hipMemcpy(dbi, bi.data(), P.nb, hipMemcpyHostToDevice));
-        PF_HIP(hipMemcpy(dbj, bj.data(), P.nb, hipMemcpyHostToDevice));
-        PF_HIP(hipMemcpy(dh, hv.data(), P.nb * sizeof(double), hipMemcpyHostToDevice));
-      }
-      hipLaunchKernelGGL(chain_conn_kernel, dim3(nb), dim3(256), 0, st, op->states, op->binom,
-                         op->diag, row0, nrows, P.nb, dbi, dbj, dh, max_per_row, cols, vals,
-                         counts, diag);
+      const bool part = order == 1;
+      const int64_t nscan = part ? op->D : nrows;
+      const unsigned nb = (unsigned)((nscan + 255) / 256);
+      const uint32_t* states = part ? op->pstates : op->states;
+      const void* tab = part ? op->prtab : op->rtab;
+      const double* dg = part ? op->pdiag : op->diag;
+      const uint32_t* perm = part ? op->pperm : nullptr;
+      if (P.tsplit > 0)
+        hipLaunchKernelGGL(chain_conn_kernel<true>, dim3(nb), dim3(256), 0, st, P, states,
+                           op->binom, tab, dg, perm, row0, nrows, nscan, max_per_row, cols, vals,
+                           counts, diag);
+      else
+        hipLaunchKernelGGL(chain_conn_kernel<false>, dim3(nb), dim3(256), 0, st, P, states,
+                           op->binom, tab, dg, perm, row0, nrows, nscan, max_per_row, cols, vals,
+                           counts, diag);
       PF_HIP(hipGetLastError());
       PF_HIP(hipStreamSynchronize(st));
-      (void)hipFree(dbi);
-      (void)hipFree(dbj);
-      (void)hipFree(dh);
     } else if (op->kind == kPacked) {
       packed_connectivity(op, row0, nrows, max_per_row, cols, vals, counts, diag, st);
     } else {
-      hipLaunchKernelGGL(qsun_conn_kernel, dim3(nb), dim3(256), 0, st, op->qsun, row0, nrows,
-                         max_per_row, cols, vals, counts, diag);
+      const unsigned nb = (unsigned)((nrows + 255) / 256);
+      hipLaunchKernelGGL(qsun_conn_kernel, dim3(nb), dim3(256), 0, st, op->qsun, op->qdiag,
+                         op->qlb, row0, nrows, max_per_row, cols, vals, counts, diag);
       PF_HIP(hipGetLastError());
       PF_HIP(hipStreamSynchronize(st));
     }
   });
+}
+
+extern "C" int pf_op_connectivity(pf_op* op, int64_t row0, int64_t nrows, int32_t max_per_row,
+                                  int64_t* cols, double* vals, int32_t* counts, double* diag,
+                                  pf_stream stream) {
+  return pf_op_connectivity_order(op, 0, row0, nrows, max_per_row, cols, vals, counts, diag,
+                                  stream);
 }
 
 // =========================================================== C ABI: maps

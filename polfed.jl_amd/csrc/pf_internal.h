@@ -82,18 +82,16 @@ struct DevBuf {
 
 // ----------------------------------------------------------- operator data
 constexpr int kMaxBonds = 128;
-constexpr int kMaxClasses = 8;
 constexpr int kMaxLoc = 27;
 constexpr int kMaxGrain = 4;
 
 // Spin-chain operator parameters, passed by value (kernel argument segment).
 struct ChainParams {
-  int32_t L, Lup, nb, ncls;
+  int32_t L, Lup, nb;
   int64_t D;
-  int32_t cls_begin[kMaxClasses + 1];  // bonds grouped by off-diagonal value class
+  // bonds with a nonzero off-diagonal value, grouped by value (ascending)
   int8_t lo[kMaxBonds], hi[kMaxBonds];  // 0-based bit positions, lo < hi
-  double cval[kMaxClasses];             // unscaled off-diagonal value J/2 of each class
-  double bval[kMaxBonds];               // cval of each bond's class
+  double bval[kMaxBonds];               // unscaled off-diagonal value J/2 of the bond
   // Two-table rank: rank(s) = TA[s & (2^m - 1)] + TB[s >> m] (m = tsplit,
   // 0 = tables not built, the kernels then use the O(1) neighbour update).
   int32_t tsplit;

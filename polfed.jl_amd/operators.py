@@ -173,9 +173,13 @@ class Operator:
             done[(dtype, ncols)] = self.autotune(ncols, dtype, nsteps)
         return done[(dtype, ncols)]
 
-    def connectivity(self, row0: int = 0, nrows: int | None = None, max_per_row: int = 64):
+    def connectivity(self, row0: int = 0, nrows: int | None = None, max_per_row: int = 64,
+                     order: int = 0):
         """Per-row off-diagonal (cols, vals, counts) and the unscaled diagonal,
-        as computed by the device kernels (host numpy arrays)."""
+        as computed by the device kernels (host numpy arrays): the step
+        kernels' rank function and diagonal table.  ``order`` 1 takes the
+        spin-chain XCD partition order's tables (rows and columns mapped back
+        to the reference order)."""
         if nrows is None:
             nrows = self.D - row0
         dev = self.device.torch_device
@@ -184,8 +188,9 @@ class Operator:
         counts = torch.empty(nrows, dtype=torch.int32, device=dev)
         diag = torch.empty(nrows, dtype=torch.float64, device=dev)
         vp = lambda t: ctypes.c_void_p(t.data_ptr())
-        check(self.lib.pf_op_connectivity(self.op, row0, nrows, max_per_row, vp(cols), vp(vals),
-                                          vp(counts), vp(diag), self.device.stream))
+        check(self.lib.pf_op_connectivity_order(self.op, int(order), row0, nrows, max_per_row,
+                                                vp(cols), vp(vals), vp(counts), vp(diag),
+                                                self.device.stream))
         return (cols.cpu().numpy(), vals.cpu().numpy(), counts.cpu().numpy(),
                 diag.cpu().numpy())
 

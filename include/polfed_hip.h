@@ -179,6 +179,19 @@ int pf_filter(pf_op* op, int32_t dtype, int32_t polytype, const double* coeffs, 
               const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t ncols,
               pf_stream stream);
 
+/* pf_filter without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * kernels and results as pf_filter; the four row-interleaved work blocks of a
+ * column chunk live in a fixed per-GPU device array of 128 MiB (allocated with
+ * the code object, held until the process ends), so 4 * D * min(ncols, 32 for
+ * FP64 / 64 for FP32) * element size must be <= 128 MiB, else PF_ERR_INVALID.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (several streams or host threads) race.  Requires pf_create on the device. */
+int pf_filter_noalloc(pf_op* op, int32_t dtype, int32_t polytype, const double* coeffs,
+                      int32_t order, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                      int32_t ncols, pf_stream stream);
+
 /* ------------------------------------------------------------- tuning */
 
 enum {

@@ -95,6 +95,8 @@ SIGNATURES = {
     "pf_clenshaw_finalsum": (_c.c_int, [_vp, _i32, _vp, _vp, _dbl, _dbl, _dbl, _dbl, _vp, _vp,
                                         _i64, _i32, _vp]),
     "pf_filter": (_c.c_int, [_vp, _i32, _i32, _pdbl, _i32, _vp, _i64, _vp, _i64, _i32, _vp]),
+    "pf_filter_noalloc": (_c.c_int, [_vp, _i32, _i32, _pdbl, _i32, _vp, _i64, _vp, _i64, _i32,
+                                     _vp]),
     "pf_op_set_tuning": (_c.c_int, [_vp, _i32, _i32]),
     "pf_autotune": (_c.c_int, [_vp, _i32, _i32, _i32, _pdbl, _pi32, _pi32, _vp]),
     "pf_fill_probes": (_c.c_int, [_i32, _u64, _i64, _i32, _vp, _i64, _vp]),

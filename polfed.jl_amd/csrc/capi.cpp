@@ -30,6 +30,12 @@ extern "C" int pf_create(pf_handle** out, int device) {
       delete h;
       throw Error{PF_ERR_HIP, "hipGetDeviceProperties failed"};
     }
+    try {
+      noalloc_scratch_init(device);  // pf_filter_noalloc's per-device work array
+    } catch (...) {
+      delete h;
+      throw;
+    }
     *out = h;
   });
 }

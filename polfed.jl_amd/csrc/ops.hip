@@ -17,6 +17,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "pf_device.h"
 #include "pf_internal.h"
@@ -1203,57 +1205,129 @@ static const uint32_t* driver_perm(const pf_op* op, const Layout& lay) {
 }
 }  // namespace pf
 
+namespace pf {
+
+// The filter driver shared by pf_filter and pf_filter_noalloc.  `work(bytes)`
+// returns the device memory of one column chunk's four row-interleaved blocks
+// (packed X, b1, b2, b3); nothing in them needs to start at zero, because the
+// first two steps read neither b2 nor b3 (zero tracking below).
+template <class Work>
+static void filter_body(pf_op* op, int32_t dtype, int32_t polytype, const double* coeffs,
+                        int32_t order, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                        int32_t ncols, hipStream_t st, Work&& work) {
+  double phi0, phi1, a_tmp, b_tmp;
+  poly_phi(polytype, phi0, phi1);
+  poly_props(polytype, 1, a_tmp, b_tmp);
+  const Affine af = op->affine(true);
+  const size_t esz = dtype == PF_F64 ? 8 : 4;
+  const int cmax = chunk_cols(dtype);
+  for (int c0 = 0; c0 < ncols; c0 += cmax) {
+    const int s = std::min(cmax, ncols - c0);
+    const Layout lay = tuned_layout(op, dtype, s);
+    const size_t blk = (size_t)op->D * s * esz;
+    char* base = reinterpret_cast<char*>(work(4 * blk));
+    void* Xp = base;
+    void* b[3] = {base + blk, base + 2 * blk, base + 3 * blk};
+    pack_block(dtype, X + (int64_t)c0 * ldx, ldx, Xp, op->D, s, nullptr, st,
+               driver_perm(op, lay));
+    double* Yc = Y + (int64_t)c0 * ldy;
+    bool z2 = true, z3 = true;  // b2 / b3 known to be zero
+    for (int k = order; k >= 1; --k) {
+      double alpha, beta_k1, dummy;
+      poly_props(polytype, k, alpha, dummy);
+      poly_props(polytype, k + 1, dummy, beta_k1);
+      int flags = kHasX | driver_part(op, lay);
+      if (!z2) flags |= kHasH;
+      if (!z3 && beta_k1 != 0.0) flags |= kHasB3;
+      launch_step(op, dtype, flags, lay, s, af, coeffs[k], alpha, beta_k1, Xp, b[1], b[2], b[0],
+                  0, st);
+      void* t = b[0];
+      b[0] = b[2];
+      b[2] = b[1];
+      b[1] = t;
+      z3 = z2;
+      z2 = false;
+    }
+    double beta1, dummy;
+    poly_props(polytype, 1, dummy, beta1);
+    int flags = kHasX | kOutCM | driver_part(op, lay);
+    if (!z2) flags |= kHasH;
+    if (!z3 && beta1 * phi0 != 0.0) flags |= kHasB3;
+    launch_step(op, dtype, flags, lay, s, af, coeffs[0] * phi0, phi1, beta1 * phi0, Xp, b[1],
+                b[2], Yc, ldy, st);
+  }
+}
+
+static void filter_args(pf_op* op, int32_t dtype, const double* coeffs, int32_t order,
+                        const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t ncols) {
+  PF_REQUIRE(op && coeffs && X && Y, "filter: null argument");
+  PF_REQUIRE(order >= 0, "filter: order >= 0");
+  PF_REQUIRE(ncols >= 1, "filter: ncols >= 1");
+  PF_REQUIRE(ldx >= op->D && ldy >= op->D, "filter: leading dimension < D");
+  PF_REQUIRE(dtype == PF_F64 || dtype == PF_F32, "filter: dtype");
+}
+
+// ------------------------------------------------ no-allocation filter work
+// Work memory of pf_filter_noalloc: a fixed device array, so that the entry
+// point never allocates.  Each GPU the library runs on holds its own copy
+// (kNoAllocScratchBytes) until the process ends.  Host code must not use the
+// array's name as an address (it is a host-side stand-in): the device address
+// is looked up once per device by noalloc_scratch_init and kept in
+// g_scratch_ptr.  All calls on one GPU share that GPU's array, so calls that
+// overlap on one GPU (different streams or threads) race.
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+constexpr int kMaxScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static void* g_scratch_ptr[kMaxScratchDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Called by pf_create with `device` current.
+void noalloc_scratch_init(int device) {
+  PF_REQUIRE(device >= 0 && device < kMaxScratchDevices, "scratch: device index too large");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    throw Error{PF_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+  g_scratch_ptr[device] = ptr;
+}
+
+static void* noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  void* p = (device >= 0 && device < kMaxScratchDevices) ? g_scratch_ptr[device] : nullptr;
+  PF_REQUIRE(p, "filter_noalloc: no scratch address for this device (pf_create not called?)");
+  return p;
+}
+
+}  // namespace pf
+
 extern "C" int pf_filter(pf_op* op, int32_t dtype, int32_t polytype, const double* coeffs,
                          int32_t order, const double* X, int64_t ldx, double* Y, int64_t ldy,
                          int32_t ncols, pf_stream stream) {
   return guarded([&] {
-    PF_REQUIRE(op && coeffs && X && Y, "filter: null argument");
-    PF_REQUIRE(order >= 0, "filter: order >= 0");
-    PF_REQUIRE(ncols >= 1, "filter: ncols >= 1");
-    PF_REQUIRE(ldx >= op->D && ldy >= op->D, "filter: leading dimension < D");
-    PF_REQUIRE(dtype == PF_F64 || dtype == PF_F32, "filter: dtype");
-    double phi0, phi1, a_tmp, b_tmp;
-    poly_phi(polytype, phi0, phi1);
-    poly_props(polytype, 1, a_tmp, b_tmp);
-    hipStream_t st = S(stream);
-    const Affine af = op->affine(true);
+    filter_args(op, dtype, coeffs, order, X, ldx, Y, ldy, ncols);
+    filter_body(op, dtype, polytype, coeffs, order, X, ldx, Y, ldy, ncols, S(stream),
+                [&](size_t bytes) { return op->fbuf.get(bytes); });
+  });
+}
+
+extern "C" int pf_filter_noalloc(pf_op* op, int32_t dtype, int32_t polytype,
+                                 const double* coeffs, int32_t order, const double* X,
+                                 int64_t ldx, double* Y, int64_t ldy, int32_t ncols,
+                                 pf_stream stream) {
+  return guarded([&] {
+    filter_args(op, dtype, coeffs, order, X, ldx, Y, ldy, ncols);
     const size_t esz = dtype == PF_F64 ? 8 : 4;
-    const int cmax = chunk_cols(dtype);
-    for (int c0 = 0; c0 < ncols; c0 += cmax) {
-      const int s = std::min(cmax, ncols - c0);
-      const Layout lay = tuned_layout(op, dtype, s);
-      const size_t blk = (size_t)op->D * s * esz;
-      char* base = reinterpret_cast<char*>(op->fbuf.get(4 * blk));
-      void* Xp = base;
-      void* b[3] = {base + blk, base + 2 * blk, base + 3 * blk};
-      pack_block(dtype, X + (int64_t)c0 * ldx, ldx, Xp, op->D, s, nullptr, st,
-                 driver_perm(op, lay));
-      double* Yc = Y + (int64_t)c0 * ldy;
-      bool z2 = true, z3 = true;  // b2 / b3 known to be zero
-      for (int k = order; k >= 1; --k) {
-        double alpha, beta_k1, dummy;
-        poly_props(polytype, k, alpha, dummy);
-        poly_props(polytype, k + 1, dummy, beta_k1);
-        int flags = kHasX | driver_part(op, lay);
-        if (!z2) flags |= kHasH;
-        if (!z3 && beta_k1 != 0.0) flags |= kHasB3;
-        launch_step(op, dtype, flags, lay, s, af, coeffs[k], alpha, beta_k1, Xp, b[1], b[2], b[0],
-                    0, st);
-        void* t = b[0];
-        b[0] = b[2];
-        b[2] = b[1];
-        b[1] = t;
-        z3 = z2;
-        z2 = false;
-      }
-      double beta1, dummy;
-      poly_props(polytype, 1, dummy, beta1);
-      int flags = kHasX | kOutCM | driver_part(op, lay);
-      if (!z2) flags |= kHasH;
-      if (!z3 && beta1 * phi0 != 0.0) flags |= kHasB3;
-      launch_step(op, dtype, flags, lay, s, af, coeffs[0] * phi0, phi1, beta1 * phi0, Xp, b[1],
-                  b[2], Yc, ldy, st);
-    }
+    const size_t need = 4 * (size_t)op->D * (size_t)std::min(ncols, chunk_cols(dtype)) * esz;
+    if (need > kNoAllocScratchBytes)
+      throw Error{PF_ERR_INVALID,
+                  "filter_noalloc: 4 x D x columns x element size = " + std::to_string(need) +
+                      " bytes exceeds the " + std::to_string(kNoAllocScratchBytes >> 20) +
+                      " MiB no-allocation scratch (use pf_filter)"};
+    char* base = reinterpret_cast<char*>(noalloc_scratch(op->h->device));
+    filter_body(op, dtype, polytype, coeffs, order, X, ldx, Y, ldy, ncols, S(stream),
+                [&](size_t) { return (void*)base; });
   });
 }
 

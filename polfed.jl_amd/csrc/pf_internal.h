@@ -237,6 +237,10 @@ void launch_packed(pf_op* op, int dtype, int flags, const Layout& lay, int ncols
                    void* out, int64_t out_ld, hipStream_t st, double* dot_part, const void* P,
                    int dot_blocks);
 
+// Looks up this device's address of pf_filter_noalloc's scratch array once per
+// device (ops.hip); called by pf_create with the device current.
+void noalloc_scratch_init(int device);
+
 // Connectivity rows of a packed operator (packed.hip).
 void packed_connectivity(pf_op* op, int64_t row0, int64_t nrows, int32_t max_per_row,
                          int64_t* cols, double* vals, int32_t* counts, double* diag,

@@ -111,7 +111,11 @@ class TransformConfig:
     (every filter in FP32: eigenvalues at FP64 accuracy, residuals limited by
     FP32 rounding), or "mixed" (FP64 until the wanted Ritz pairs' residual
     estimate drops below ``switch_tol``, then FP32, falling back to FP64 if the
-    final residuals miss eigentol)."""
+    final residuals miss eigentol).  With ``polish`` (default) the pairs of a
+    run that used FP32 filters are extracted by the FP64 polish of
+    ``refine.polish`` (an FP64 window filter of degree ``polish_fraction`` x
+    order on the Ritz vectors, then Rayleigh-Ritz with the unscaled H).
+    Extensions of the reference, which filters in FP64 only."""
 
     coefficients: Callable = PolfedDefaults.coefficients
     normalization: float = PolfedDefaults.normalization
@@ -122,7 +126,9 @@ class TransformConfig:
     order_safety_factor: float = PolfedDefaults.order_safety_factor
     polynomialtype: str = PolfedDefaults.polynomialtype
     dtype: str = "f64"
-    switch_tol: float = 1e-6
+    switch_tol: float = 1e-4
+    polish: bool = True
+    polish_fraction: float = 1.0 / 16.0
 
 
 class MatrixBasis:

@@ -25,12 +25,17 @@ import numpy as np
 import torch
 
 from . import blockops as bo
+from . import refine
 from . import spectral as spc
 from .configs import (DoSConfig, FactorizationConfig, MappingConfig, PolfedDefaults,
                       TransformConfig)
 from .device import Device, default_device
 from .lanczos import LanczosResult, lanczos
 from .operators import Operator, PackedOperator
+
+
+# widest Lanczos block of the device path (pf_block_qr, pf_reorth, pf_rotate)
+MAX_BLOCK = bo.MAX_N
 
 
 @dataclass
@@ -187,6 +192,28 @@ def _filter_callable(op: Operator, mapping: MappingConfig, coeffs: np.ndarray, p
     return apply
 
 
+def _polish_callable(dev: Device, op: Operator, mapping: MappingConfig,
+                     transform: TransformConfig, prep: "_Prepared", which: str):
+    """The FP64 polish of a run with FP32 filters (refine.polish), or None
+    (FP64 filters, polish off, or user mapping hooks: the library's own
+    operator is then not the filtered one)."""
+    if transform.dtype == "f64" or not transform.polish or not isinstance(op, Operator):
+        return None
+    if mapping.clenshaw_recurrence is not None or mapping.clenshaw_finalsum is not None \
+            or mapping.f_rescaled is not None:
+        return None
+    plan = prep.plan
+
+    def polish(Y, keep):
+        vals, vecs, res, _ = refine.polish(dev, op, Y, keep, target=plan.target,
+                                           order=plan.order, coeffs=prep.coeffs,
+                                           polytype=plan.polynomialtype, which=which,
+                                           fraction=transform.polish_fraction)
+        return vals, vecs, res
+
+    return polish
+
+
 @dataclass
 class _Prepared:
     """Per-problem state between the set-up stages and the Lanczos stage."""
@@ -209,6 +236,10 @@ def _prepare(op: Operator, x0, howmany: int, target, mapping: MappingConfig,
     mapping plan with the Emin/Emax probes, KPM DoS, interval and order, and
     the normalized filter coefficients."""
     times = {}
+    if getattr(x0, "ndim", 1) == 2 and min(x0.shape) > MAX_BLOCK:
+        raise ValueError(f"POLFED on the GPU supports blocks of at most {MAX_BLOCK} columns "
+                         f"(the block QR and reorthogonalization kernels); got "
+                         f"{min(x0.shape)}.")
     x0 = prepare_initial_state(dev, x0)
     s = 1 if x0.dim() == 1 else x0.shape[0]
     D = x0.shape[-1]
@@ -363,7 +394,8 @@ def polfed(op: Operator, x0, howmany: int, target, *, produce_report: bool = Fal
     res = lanczos(dev, f_t, prep.x0, howmany, maxdim=prep.maxdim, which=fact.which, tol=fact.tol,
                   eigentol=fact.eigentol if fact.eigentol is not None else 1e-8, mapvals=op,
                   rot=fact.rot, f_fast=f_fast, fast_mode=fast_mode,
-                  switch_tol=transform.switch_tol, basistype=fact.basistype)
+                  switch_tol=transform.switch_tol, basistype=fact.basistype,
+                  polish=_polish_callable(dev, op, mapping, transform, prep, fact.which))
     torch.cuda.synchronize()
     times["lanczos"] = time.perf_counter() - t0
     times.update({f"lanczos.{k}": v for k, v in res.times.items()})
@@ -446,7 +478,8 @@ def polfed_batch(ops, x0s, howmany: int, target, *, produce_report: bool = False
     gens = [lanczos_coroutine(dev, p.x0, howmany, maxdim=p.maxdim, which=fact.which, tol=fact.tol,
                               eigentol=eigentol, mapvals=op, rot=fact.rot, stage_timers=False,
                               has_fast=transform.dtype != "f64", fast_mode=fast_mode,
-                              switch_tol=transform.switch_tol, basistype=fact.basistype)
+                              switch_tol=transform.switch_tol, basistype=fact.basistype,
+                              polish=_polish_callable(dev, op, mapping, transform, p, fact.which))
             for op, p in zip(ops, preps)]
     t0 = time.perf_counter()
     pending = {i: next(g) for i, g in enumerate(gens)}

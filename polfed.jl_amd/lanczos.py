@@ -126,7 +126,7 @@ class _Timer:
 def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | None = None,
             which: str = "SR", tol: float = 1e-14, eigentol: float = 1e-8, mapvals=None,
             rot=None, stage_timers: bool = True, f_fast=None, fast_mode: str = "never",
-            switch_tol: float = 1e-6, basistype=None) -> LanczosResult:
+            switch_tol: float = 1e-6, basistype=None, polish=None) -> LanczosResult:
     """Block Lanczos for ``howmany`` pairs of the operator ``f``.
 
     ``f(X, Y)`` writes the operator applied to the (s, D) block X into Y.
@@ -144,11 +144,19 @@ def lanczos(dev: Device, f, x0: torch.Tensor, howmany: int, *, maxdim: int | Non
 
     ``basistype`` selects the basis storage (``MatrixBasis`` default,
     ``HybridMatrixBasis``: see basis.py).
+
+    ``polish(Y, keep) -> (vals, vecs, res)`` (refine.polish) replaces the
+    plain Ritz extraction when some filter applications ran in the fast form:
+    it receives the Ritz vectors of the Lanczos-converged pairs, best first,
+    and returns as many pairs in the same order.  (Unconverged Ritz vectors
+    are not added as a guard set: Rayleigh-Ritz on such mixtures yields
+    spurious interior Ritz values that rank among the wanted ones.)
     """
     gen = lanczos_coroutine(dev, x0, howmany, maxdim=maxdim, which=which, tol=tol,
                             eigentol=eigentol, mapvals=mapvals, rot=rot,
                             stage_timers=stage_timers, has_fast=f_fast is not None,
-                            fast_mode=fast_mode, switch_tol=switch_tol, basistype=basistype)
+                            fast_mode=fast_mode, switch_tol=switch_tol, basistype=basistype,
+                            polish=polish)
     try:
         X, Y, fast = next(gen)
         while True:
@@ -178,7 +186,7 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
                       maxdim: int | None = None, which: str = "SR", tol: float = 1e-14,
                       eigentol: float = 1e-8, mapvals=None, rot=None, stage_timers: bool = True,
                       has_fast: bool = False, fast_mode: str = "never",
-                      switch_tol: float = 1e-6, basistype=None):
+                      switch_tol: float = 1e-6, basistype=None, polish=None):
     """The block Lanczos of ``lanczos`` as a generator: it yields every
     operator application it needs as ``(X, Y, fast)`` -- write f(X) (or
     f_fast(X) when ``fast``) into Y, then resume it with ``send(None)`` -- and
@@ -244,13 +252,22 @@ def lanczos_coroutine(dev: Device, x0: torch.Tensor, howmany: int, *,
         hist_k.append(k)
         hist_r.append(maxres)
         if converged == howmany or final:
-            Phi = phi[:, idxs[:converged]]
-            vecs = torch.empty((converged, D), dtype=torch.float64, device=X0.device)
-            if converged:
-                basis.combine(_to_dev_matrix(Phi, r), vecs, k)
-                vals, res_out = _eigen_residuals(dev, mapvals, vecs)
+            if converged and polish is not None and fast_iters > 0:
+                # FP32 products in the basis: FP64 polish of the converged
+                # pairs (refine.polish)
+                Y = torch.empty((converged, D), dtype=torch.float64, device=X0.device)
+                basis.combine(_to_dev_matrix(phi[:, idxs[:converged]], r), Y, k)
+                with tm("polish"):
+                    vals, vecs, res_out = polish(Y, converged)
+                del Y
             else:
-                vals, res_out = np.zeros(0), np.zeros(0)
+                Phi = phi[:, idxs[:converged]]
+                vecs = torch.empty((converged, D), dtype=torch.float64, device=X0.device)
+                if converged:
+                    basis.combine(_to_dev_matrix(Phi, r), vecs, k)
+                    vals, res_out = _eigen_residuals(dev, mapvals, vecs)
+                else:
+                    vals, res_out = np.zeros(0), np.zeros(0)
             eigenconverged = 0
             for rv in res_out:
                 if rv < eigentol:

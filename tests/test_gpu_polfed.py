@@ -151,6 +151,49 @@ def test_polfed_mixed_precision_meets_residual_tolerance(dev):
     assert lz.all_residuals.max() < 1e-3
 
 
+def test_fp32_filter_polish(dev):
+    """The FP64 polish of runs whose Krylov basis holds FP32-filter products:
+    a window filter in FP64 and Rayleigh-Ritz with the unscaled H.  With an
+    early switch to FP32 (and with FP32 throughout) the polished pairs have
+    residuals at least 10x below the plain Ritz pairs' (recomputed here with
+    the oracle's H), orthonormal vectors, Rayleigh quotients equal to the
+    returned values, and every pair meeting 1e-10 matches an eigenvalue of the
+    dense spectrum to 1e-10 x width."""
+    from polfed_amd import FactorizationConfig, MappingConfig, TransformConfig, polfed
+    from polfed_amd.blockops import fill_probes
+    from polfed_amd.models import xxz_hamiltonian
+    L, nev, s = 14, 24, 4
+    h = om.resolve_fields(L, 0.0, 5.0, rng=np.random.default_rng(2))
+    op = xxz_hamiltonian(L, L // 2, 1.0, 1.0, 5.0, fields=h, device=dev)
+    H = om.xxz_hamiltonian(L, L // 2, 1.0, 1.0, 5.0, fields=h)
+    ev = np.linalg.eigvalsh(H.toarray())
+    width = ev[-1] - ev[0]
+    q, _ = np.linalg.qr(np.random.default_rng(7).standard_normal((op.D, s)))
+    P = fill_probes(dev, op.D, 100, seed=5)
+    common = dict(produce_report=True, mapping=MappingConfig(Emin=ev[0], Emax=ev[-1]),
+                  fact=FactorizationConfig(eigentol=1e-10), probes=P)
+    for mode, sw, gain in (("mixed", 1e-2, 10.0), ("f32", 1e-6, 3.0)):
+        res = {}
+        for polish in (False, True):
+            tc = TransformConfig(dtype=mode, switch_tol=sw, polish=polish)
+            _, _, rep = polfed(op, q, nev, "middle", transform=tc, **common)
+            lz = rep.lanczos
+            assert lz.fast_iterations > 0
+            res[polish] = (lz.all_vals, lz.all_residuals)
+        vals, r = res[True]
+        assert np.max(r) * gain <= np.max(res[False][1]), mode
+        # the returned pairs, checked on the host with the oracle's H
+        vals_p, psi, rep = polfed(op, q, nev, "middle",
+                                  transform=TransformConfig(dtype=mode, switch_tol=sw), **common)
+        if len(vals_p):
+            psi = dev.to_host(psi)
+            Hpsi = H @ psi
+            assert np.linalg.norm(Hpsi - psi * vals_p, axis=0).max() <= 1e-10
+            assert np.max(np.abs(np.einsum("ij,ij->j", psi, Hpsi) - vals_p)) <= 1e-12 * width
+            assert np.linalg.norm(psi.T @ psi - np.eye(len(vals_p))) <= 1e-12
+            assert max(np.min(np.abs(ev - v)) for v in vals_p) <= 1e-10 * width
+
+
 def test_lanczos_extrema_and_report(dev):
     """lanczos_extrema (src/Lanczos/Lanczos.jl:146-212; test/test_polfed.jl:94-107)
     on a matrix, a GPU operator and a callable: the extremal eigenvalues, the
@@ -180,3 +223,14 @@ def test_lanczos_extrema_and_report(dev):
                        transform=TransformConfig(order=60), dos=DoSConfig(N=32, R=8))
     text = display_report(rep)
     assert "order 60" in text and "iterations" in text
+
+
+def test_polfed_rejects_blocks_wider_than_64(dev):
+    """Blocks wider than the device block kernels take fail before any work,
+    with a clear error (not inside the Lanczos loop after the DoS)."""
+    from polfed_amd import polfed
+    from polfed_amd.models import xxz_hamiltonian
+    op = xxz_hamiltonian(10, 5, 1.0, 1.0, 1.0, rng=np.random.default_rng(3), device=dev)
+    x0 = np.random.default_rng(4).standard_normal((op.D, 65))
+    with pytest.raises(ValueError, match="at most 64 columns"):
+        polfed(op, x0, 70, "middle")

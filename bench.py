@@ -1,16 +1,28 @@
 #!/usr/bin/env python3
 """POLFED hot-path benchmark on one MI355X.
 
-Workload: random-field XXZ chain, L=20 periodic, Sz=0 sector (D=184756),
-J=1, Delta=1, fields h_i ~ U[-3,3] from numpy seed 1, block size 8, nev=100,
-target = middle of the spectrum, reference defaults (cutoff 0.17, DoS N=250,
-R=300 Jackson), eigentol 1e-10.  One step = one complete POLFED solve (time to
-nev converged eigenpairs with ||H psi - E psi|| <= 1e-10), inputs resident on
-the GPU.  The filter's achieved bandwidth is measured with HIP events around
-every filter call inside the timed region.
+Default workload (the precision configuration): random-field XXZ chain, L=22
+periodic, Sz=0 sector (D=705432), J=1, Delta=1, fields h_i ~ U[-2,2] from
+numpy seed 2, block size 16, nev=200, target = middle of the spectrum,
+reference defaults (cutoff 0.17, DoS N=250, R=300 Jackson), eigentol 1e-10,
+inputs resident on the GPU.  In one run:
 
-Prints one JSON line (see the keys below).  --gpus N > 1 is not supported:
-this build targets a single GPU.
+* time to nev eigenpairs (||H psi - E psi|| <= 1e-10) of the plain FP64-filter
+  solve and of the FP32-filter form (mixed FP64/FP32 filters + FP64 polish),
+  same start block and KPM probes, every returned pair's eigenvalue error
+  against the FP64 run and its residual (``time_to_nev``);
+* the timed steps: each step is one FP32 Clenshaw filter application of the
+  solve's polynomial (order K, K+1 fused step launches) on the 16-column
+  block; ``value`` is their algorithmic bandwidth (4 x 16 x D x 4 bytes per
+  launch), the roofline entry that of the FP32 step kernel from HIP events;
+* FP32 against FP64 KPM moments on the same probes;
+* the CPU baseline (the oracle's C restatement of the reference's CPU filter,
+  a bounded sample projected to the FP64 solve) and the reference's
+  CPU-runnable case (XXZ L=14, single vector) solved in full on CPU and GPU.
+
+``--workload xxz-l20`` keeps the former bench line (one FP64 solve per step).
+Prints one JSON line.  --gpus N > 1 is not supported: this build targets a
+single GPU.
 """
 from __future__ import annotations
 
@@ -112,28 +124,30 @@ def progress_filter(op, label, every=5):
 
 
 def kernel_name(op, dtype="f64", s=8):
-    """Template instance of the step kernel the autotuner picked for (dtype, s)."""
-    tuned = getattr(op, "_tuned", {}).get((dtype, s))
-    if tuned is None:
-        return None
-    V = tuned[1]
-    return f"chain_step_kernel<{'double' if dtype == 'f64' else 'float'}, {V}, {s // V}>"
+    """Template instance of the spin-chain step kernel the drivers launch for
+    (dtype, s) -- the autotuned layout -- and that layout as the --layout
+    string (V,PBLOCKS,WIDE,PARTITION) of the filter-only profiling driver."""
+    t = op.tuning(s, dtype)
+    T = "double" if dtype == "f64" else "float"
+    V, G = t["vec"], s // t["vec"]
+    if t["occupancy"]:
+        name = f"chain_step_kernel_occ<{T}, {V}, {G}>"
+    else:
+        name = f"chain_step_kernel<{T}, {V}, {G}, true, {2 if t['wide'] else 1}>"
+    return name, f"{V},{t['pblocks']},{t['wide']},{t['row_partition']}"
 
 
 def measured_traffic(kernel):
-    """(full kernel name, HBM-side bytes per launch) of the step kernel the
-    autotuner picked, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary (each counter scaled by its plain-copy calibration factor), when
-    the counted instance has the same element type and lanes layout as
-    `kernel`; (kernel, None) otherwise."""
+    """HBM-side bytes per launch of the step kernel instance `kernel` (full
+    template name) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (each counter scaled by its plain-copy calibration factor), or
+    None when that instance was not counted."""
     try:
-        table = json.loads(TRAFFIC_FILE.read_text())
-        sel = table["selected"]
+        table = json.loads(TRAFFIC_FILE.read_text())["kernels"]
     except (OSError, ValueError, KeyError):
-        return kernel, None
-    if kernel and sel["kernel"].startswith(kernel[:-1] + ","):
-        return sel["kernel"], float(sel["bytes_per_launch"])
-    return kernel, None
+        return None
+    entry = table.get(kernel)
+    return float(entry["bytes_per_launch"]) if entry else None
 
 
 def filter_only(args):
@@ -188,8 +202,9 @@ def filter_only(args):
     torch.cuda.synchronize()
     us = e0.elapsed_time(e1) * 1e3 / (reps * (K + 1))
     esz = 8 if dt == "f64" else 4
-    print(json.dumps({"mode": "filter-only", "kernel": kernel_name(op, dt, s),
-                      "layout": args.layout or "autotuned", "dtype": dt,
+    kname, klayout = kernel_name(op, dt, s)
+    print(json.dumps({"mode": "filter-only", "kernel": kname,
+                      "layout": klayout, "forced": bool(args.layout), "dtype": dt,
                       "D": op.D, "s": s, "steps": reps * (K + 1), "us_per_step": us,
                       "algorithmic_GBps": 4 * s * op.D * esz / (us * 1e-6) / 1e9}), flush=True)
 
@@ -351,8 +366,16 @@ def qsun_batch(args):
         conv = [int(len(v)) for v, _, _ in res_list]
         maxres = max(float(r.lanczos.all_residuals.max()) if r.lanczos.all_residuals is not None
                      and r.lanczos.all_residuals.size else 0.0 for _, _, r in res_list)
+        # instances that stop short of nev: the reference's Krylov budget
+        # (expectedkrylovdim, maxiter iterations) ran out first
+        short = [i for i, c in enumerate(conv) if c < 32]
         return {"time_s": t, "pairs_total": int(sum(conv)), "instances_complete":
-                int(sum(c == 32 for c in conv)), "max_residual": maxres,
+                int(sum(c == 32 for c in conv)), "instances_short_of_nev": short,
+                "pairs_of_short_instances": [conv[i] for i in short],
+                "short_at_iteration_budget": all(res_list[i][2].iterations
+                                                 >= res_list[i][2].lanczos.maxiter
+                                                 for i in short),
+                "max_residual": maxres,
                 "iterations_max": int(max(r.iterations for _, _, r in res_list)),
                 "orders": [int(r.order) for _, _, r in res_list]}
 
@@ -403,52 +426,259 @@ def qsun_batch(args):
     print(json.dumps(out), flush=True)
 
 
-def xxz_l22(args):
-    """Precision workload: XXZ L=22 Sz=0 (D=705432), W=2 (numpy seed 2),
-    block 16, nev=200, target middle, eigentol 1e-10.  The FP64-filter solve,
-    then the mixed and pure-FP32 filter solves from the same start block (same
-    probes, so the same order K); every eigenvalue and residual is reported
-    against the FP64 run."""
+def cpu_info():
+    """Host CPU model, logical CPUs and the CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
+
+
+def precision_workload(dev):
+    """XXZ L=22 Sz=0 (D=705432), J=1, Delta=1, fields h_i = 2(2u-1) with u from
+    numpy seed 2, start block = QR of a numpy seed-102 normal D x 16 block."""
     import numpy as np
-    import torch
-    from polfed_amd import FactorizationConfig, TransformConfig, models, polfed
-    from polfed_amd.device import Device
-    dev = Device(0)
+    from polfed_amd import models
     op = models.xxz_hamiltonian(22, 11, 1.0, 1.0, 2.0, rng=np.random.default_rng(2), device=dev)
     q, _ = np.linalg.qr(np.random.default_rng(102).standard_normal((op.D, 16)))
-    x0 = dev.to_device(q)
-    out = {"metric": METRIC, "workload": "XXZ L=22 Sz=0 D=705432 W=2 seed 2, block 16, "
-           "nev=200, target middle, eigentol 1e-10", "unit": "s", "higher_is_better": False}
-    ref = None
+    return op, dev.to_device(q), q
+
+
+def _pair_errors(vals, res, ref_vals, width):
+    """Each returned pair's eigenvalue error against the nearest FP64-run
+    eigenvalue (over the spectral width) and its residual."""
+    import numpy as np
+    ref = np.sort(ref_vals)
+    err = []
+    for v in vals:
+        j = int(np.searchsorted(ref, v))
+        cand = [ref[i] for i in (j - 1, j) if 0 <= i < ref.size]
+        err.append(min(abs(v - c) for c in cand) / width if cand else float("nan"))
+    return np.array(err), np.asarray(res)
+
+
+def cpu_baseline_c3(op, rep, x0_host, budget_s=8.0):
+    """The oracle's C restatement of the reference's packed CPU filter on a
+    bounded prefix of one filter application of the solve's polynomial on the
+    16-column block, projected to the FP64 solve's (K+1) x iterations: once
+    with one thread per column (MulColsParallel(1), the reference's default
+    for a block, ParallelizationClenshaw.jl:77-95) and once with every usable
+    core splitting each column's rows (MulColsParallel(nt), make_loop.jl)."""
+    import numpy as np
+    from oracle import cpu as ocpu
+    from oracle import models as om
+    from oracle import packing as opk
+    from polfed_amd import spectral as spc
+    H = om.spin_half_hamiltonian(op.L, op.bonds, op.couplings, op.anisotropies, op.fields,
+                                 op.Lup)
+    d, bk = opk.get_diags_and_offdiagonals_by_value(H)
+    dr, bkr = opk.rescaled_packing(d, bk, rep.Emin, rep.Emax)
+    P = ocpu.PackedCPU(dr, bkr)
+    s = x0_host.shape[1]
+    info = cpu_info()
+    cores = info["usable_cpus"]
+    c = spc.normalized_coefficients(spc.PolfedDefaults.coefficients, rep.target, rep.order)
+    steps_app = rep.order + 1
+    out = dict(info)
+    # (a) one thread per column
+    thr = min(s, cores)
+    P.filter(c[:21], x0_host, nthreads=thr)  # warm-up: threads, first touch
+    t0 = time.perf_counter()
+    P.filter(c[:101], x0_host, nthreads=thr)
+    est = (time.perf_counter() - t0) / 101
+    n = int(min(steps_app, max(101, budget_s / est)))
+    t0 = time.perf_counter()
+    P.filter(c[:n], x0_host, nthreads=thr)
+    per_step_cols = (time.perf_counter() - t0) / n
+    # (b) all usable cores on each column's rows, columns in turn
+    x1 = np.ascontiguousarray(x0_host[:, 0])
+    P.filter_rows(c[:21], x1, nthreads=cores)
+    t0 = time.perf_counter()
+    P.filter_rows(c[:101], x1, nthreads=cores)
+    est = (time.perf_counter() - t0) / 101 * s
+    n2 = int(min(steps_app, max(101, budget_s / est)))
+    t0 = time.perf_counter()
+    P.filter_rows(c[:n2], x1, nthreads=cores)
+    per_step_rows = (time.perf_counter() - t0) / n2 * s
+    proj = lambda t: t * steps_app * rep.iterations
+    out.update({
+        "value": proj(per_step_cols), "unit": "s", "cores": thr, "kind": "port",
+        "sample": (f"oracle/cpu_ref.c packed-bucket Clenshaw filter (FP64, the reference's CPU "
+                   f"operator layout), one thread per column of the 16-column block: the first "
+                   f"{n} of {steps_app} steps of one filter application, {per_step_cols * 1e3:.3f}"
+                   f" ms/step, projected x (K+1) x {rep.iterations} Lanczos iterations of the "
+                   f"FP64 solve (filter only; projected, not run)"),
+        "all_cores": {"value": proj(per_step_rows), "cores": cores,
+                      "sample": (f"the same filter with all {cores} usable cores splitting each "
+                                 f"column's rows (columns in turn): the first {n2} steps of one "
+                                 f"column, {per_step_rows * 1e3:.3f} ms per 16-column step, "
+                                 f"projected the same way")}})
+    return out
+
+
+def c1_full_solves(dev):
+    """The reference's CPU-runnable case, run in full on both sides: XXZ L=14
+    Sz=0 (D=3432), W=5 seed 0, single vector, nev=50, target middle, reference
+    defaults, eigentol 1e-10: the oracle's complete CPU solve (one thread for
+    the sparse products) and the GPU FP64 solve, same start vector and probes."""
+    import numpy as np
+    import torch
+    from oracle import models as om
+    from oracle import polfed as opf
+    from polfed_amd import FactorizationConfig, models, polfed
+    from polfed_amd.blockops import fill_probes
+    L, nev = 14, 50
+    h = om.resolve_fields(L, 0.0, 5.0, rng=np.random.default_rng(0))
+    op = models.xxz_hamiltonian(L, L // 2, 1.0, 1.0, 5.0, fields=h, device=dev)
+    H = om.xxz_hamiltonian(L, L // 2, 1.0, 1.0, 5.0, fields=h)
+    x0 = np.random.default_rng(100).standard_normal(op.D)
+    x0 /= np.linalg.norm(x0)
+    P = fill_probes(dev, op.D, 300, seed=0x5EED)
+
+    def gpu():
+        return polfed(op, x0, nev, ("rescaled", 0.0), produce_report=True,
+                      fact=FactorizationConfig(eigentol=1e-10), probes=P)
+    gpu()  # autotuning and first launches
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vals, _, rep = gpu()
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ref = opf.polfed(lambda X: H @ X, x0, nev, ("rescaled", 0.0), eigentol=1e-10,
+                     probes=dev.to_host(P))
+    t_cpu = time.perf_counter() - t0
+    same = ref.plan.order == rep.order and len(vals) == len(ref.vals)
+    return {"workload": "XXZ L=14 Sz=0 D=3432 W=5 seed 0, single vector, nev=50, target "
+                        "middle, eigentol 1e-10 (full solves, run)",
+            "cpu_oracle_s": t_cpu, "cpu_threads": 1, "gpu_f64_s": t_gpu,
+            "order": rep.order, "cpu_order": ref.plan.order, "pairs": int(len(vals)),
+            "cpu_pairs": int(len(ref.vals)),
+            "max_eig_diff_over_width": (float(np.max(np.abs(np.sort(vals) - ref.vals))
+                                              / (rep.Emax - rep.Emin)) if same else None)}
+
+
+def precision_bench(args):
+    """The default bench: time to nev eigenpairs on the precision workload
+    (XXZ L=22, block 16, nev=200) with the FP64 filter (the plain drop-in) and
+    with the FP32-filter form, both complete solves in this run, every pair
+    of the FP32-filter form against the FP64 run; the timed steps are whole
+    FP32 filter applications of the solve's polynomial on the 16-column block
+    (the hot path), which give `value` (algorithmic GB/s) and the roofline of
+    the FP32 step kernel."""
+    import numpy as np
+    import torch
+    from polfed_amd import FactorizationConfig, TransformConfig, polfed
+    from polfed_amd.blockops import fill_probes
+    from polfed_amd.device import Device
+    dev = Device(0)
+    op, x0, x0_host = precision_workload(dev)
+    D, s, nev = op.D, 16, 200
+    log(f"workload XXZ L=22 D={D} s={s} nev={nev}")
+    probes = fill_probes(dev, D, 300, seed=0x22)  # the same DoS (hence order) for both solves
+    fact = FactorizationConfig(eigentol=1e-10)
+    sw = args.switch_tol if args.switch_tol is not None else TransformConfig.switch_tol
+    solves = {}
     progress_filter(op, "l22")
-    for mode in ("f64", "mixed", "f32"):
+    for label, tc in (("fp64", TransformConfig(dtype="f64")),
+                      ("fp32_filter", TransformConfig(dtype="mixed", switch_tol=sw))):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        vals, vecs, rep = polfed(op, x0, 200, "middle", produce_report=True,
-                                 transform=TransformConfig(dtype=mode, **_switch(args)),
-                                 fact=FactorizationConfig(eigentol=1e-10))
+        vals, vecs, rep = polfed(op, x0, nev, "middle", produce_report=True, transform=tc,
+                                 fact=fact, probes=probes)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         lz = rep.lanczos
-        allv = np.sort(lz.all_vals) if lz.all_vals is not None else np.zeros(0)
-        allr = lz.all_residuals if lz.all_residuals is not None else np.zeros(0)
-        e = {"time_s": t, "order": rep.order, "iterations": rep.iterations,
-             "returned_pairs": int(len(vals)), "lanczos_converged": int(lz.converged),
-             "max_residual": float(allr.max()) if allr.size else None,
-             "median_residual": float(np.median(allr)) if allr.size else None,
-             "fp32_iterations": int(lz.fast_iterations), "fp64_fallback": bool(lz.fallback)}
-        if ref is None:
-            ref = (allv, rep.Emax - rep.Emin, t)
-        else:
-            n = min(len(allv), len(ref[0]))
-            err = np.abs(allv[:n] - ref[0][:n]) / ref[1]
-            e.update({"speedup_vs_fp64": ref[2] / t,
-                      "eig_err_over_width_max": float(err.max()) if n else None,
-                      "eig_err_over_width_median": float(np.median(err)) if n else None})
-        out[mode] = e
-        log(f"l22 {mode}: {t:.1f} s K={rep.order} iters={rep.iterations} pairs={len(vals)} "
-            f"maxres={e['max_residual']}")
-    print(json.dumps(out), flush=True)
+        solves[label] = (t, vals, lz.residuals, rep)
+        log(f"{label}: {t:.1f} s K={rep.order} iters={rep.iterations} pairs={len(vals)} "
+            f"fp32 iters={lz.fast_iterations}")
+        del vecs
+    t64, v64, r64, rep64 = solves["fp64"]
+    t32, v32, r32, rep32 = solves["fp32_filter"]
+    width = rep64.Emax - rep64.Emin
+    eerr, res32 = _pair_errors(v32, r32, v64, width)
+    lz32 = rep32.lanczos
+    ok32 = int(np.sum(res32 <= 1e-10))
+    time_to_nev = {
+        "workload": "XXZ L=22 Sz=0 D=705432 W=2 seed 2, block 16, nev=200, target middle, "
+                    "eigentol 1e-10, same start block and KPM probes for both solves",
+        "fp64": {"time_s": t64, "order": rep64.order, "iterations": rep64.iterations,
+                 "pairs": int(len(v64)), "max_residual": float(r64.max()) if r64.size else None},
+        "fp32_filter": {"time_s": t32, "mode": f"mixed (switch_tol {sw:g}) + FP64 polish",
+                        "order": rep32.order, "iterations": rep32.iterations,
+                        "fp32_iterations": int(lz32.fast_iterations),
+                        "switch_iteration": int(lz32.switch_iteration),
+                        "fp64_fallback": bool(lz32.fallback),
+                        "polish_s": float(lz32.times.get("polish", 0.0)),
+                        "pairs": int(len(v32)), "pairs_residual_le_1e-10": ok32,
+                        "max_residual": float(res32.max()) if res32.size else None},
+        "speedup_fp32_filter_vs_fp64": t64 / t32,
+        "per_pair": {"eig_err_over_width": [float(f"{e:.2g}") for e in eerr],
+                     "residual": [float(f"{r:.2g}") for r in res32]},
+    }
+    log(f"speed-up {t64 / t32:.3f}x, pairs {len(v32)} (<=1e-10: {ok32}) vs {len(v64)}")
+
+    # ---- timed hot-path steps: FP32 filter applications of the solve's polynomial
+    from polfed_amd import spectral as spc
+    c = spc.normalized_coefficients(spc.PolfedDefaults.coefficients, rep64.target, rep64.order)
+    y = torch.empty_like(x0)
+    for _ in range(args.warmup):
+        op.filter(c, x0, y, dtype="f32")
+    torch.cuda.synchronize()
+    events = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        op.filter(c, x0, y, dtype="f32")
+        e1.record()
+        events.append((e0, e1))
+    torch.cuda.synchronize()
+    t_steps = time.perf_counter() - t0
+    launches = rep64.order + 1
+    per_launch = 4 * s * D * 4
+    ms_app = sum(a.elapsed_time(b) for a, b in events) / len(events)
+    avg_launch_ms = ms_app / launches
+    kname, klayout = kernel_name(op, "f32", s)
+    traffic = measured_traffic(kname)
+    value = per_launch * launches * args.steps / t_steps / 1e9
+    achieved = per_launch / (avg_launch_ms * 1e-3) / 1e9
+    log(f"steps: {args.steps} x {t_steps / args.steps:.3f} s, {value:.0f} GB/s, kernel {kname}")
+
+    # ---- KPM moments in FP32 against FP64 on the same probes
+    mu64 = op.kpm_moments(250, probes, dtype="f64")
+    mu32 = op.kpm_moments(250, probes, dtype="f32")
+    kerr = np.abs(mu32 - mu64) / D
+
+    line = {"metric": METRIC, "value": value, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": t_steps / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (numpy seeded fields and start block)",
+            "config": {"workload": "XXZ L=22 Sz=0 D=705432 W=2 seed 2, block 16, nev=200, "
+                                   "target middle; step = one FP32 Clenshaw filter application "
+                                   f"(order {rep64.order}) on the 16-column block"},
+            "time_to_nev": time_to_nev,
+            "kpm_fp32_vs_fp64": {"moments": 250, "probes": 300,
+                                 "max_err_over_D": float(kerr.max()),
+                                 "median_err_over_D": float(np.median(kerr))},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": traffic, "kernel": kname, "layout": klayout,
+                         "per_launch_bytes": per_launch, "avg_launch_ms": avg_launch_ms}}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_c3(op, rep64, x0_host)
+        line["cpu_baseline"]["c1_full_solve"] = c1_full_solves(dev)
+    print(json.dumps(line, separators=(",", ":")), flush=True)
 
 
 def xxz_l14(args):
@@ -682,70 +912,9 @@ def cpu_baseline(op, report, x0_host, budget_s=15.0):
                        f"{report.iterations} Lanczos iterations (filter only)")}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-fp32", action="store_true",
-                    help="skip the FP32-filter solves (mixed and pure FP32)")
-    ap.add_argument("--filter-only", type=int, default=0, metavar="STEPS",
-                    help="profiling driver: time about STEPS filter steps only")
-    ap.add_argument("--dtype", default="f64", choices=["f64", "f32"],
-                    help="filter arithmetic for --filter-only")
-    ap.add_argument("--packed-compare", action="store_true",
-                    help="profiling driver: packed sparse operator vs matrix-free filter steps")
-    ap.add_argument("--layout", default="", metavar="V,PBLOCKS,WIDE,PARTITION",
-                    help="--filter-only: fixed step-kernel launch configuration instead of "
-                         "the autotuner's (e.g. 2,8,0,1)")
-    ap.add_argument("--row-partition", type=int, default=-1, choices=[-1, 0, 1],
-                    help="--filter-only: spin-chain row order (-1 autotuned, 0 reference, "
-                         "1 XCD partition)")
-    ap.add_argument("--reorth-only", type=int, default=0, metavar="CALLS",
-                    help="profiling driver: time CALLS CGS2 reorthogonalizations")
-    ap.add_argument("--workload", default="xxz-l20",
-                    choices=["xxz-l20", "xxz-l14", "qsun-batch", "xxz-l22", "j1j2-l24"],
-                    help="xxz-l20: the bench line; xxz-l14: the CPU-runnable case against the "
-                         "oracle's full CPU solve; qsun-batch: 64 Quantum-Sun instances; "
-                         "xxz-l22: FP32/FP64 precision report; j1j2-l24: KPM and filter at L=24")
-    ap.add_argument("--instances", type=int, default=64, help="qsun-batch: number of instances")
-    ap.add_argument("--modes", default="mixed,f64",
-                    help="qsun-batch: comma list of mixed, f64 (batched) and plain (FP64, "
-                         "one polfed call per instance)")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="profiling driver: one 256 MiB device copy, five times")
-    ap.add_argument("--switch-tol", type=float, default=None,
-                    help="mixed-precision solves: residual estimate at which the filter "
-                         "switches to FP32 (default: TransformConfig's)")
-    ap.add_argument("--modes-l20", default="mixed,f32",
-                    help="FP32-filter solves reported beside the FP64 bench solve")
-    ap.add_argument("--hybrid-basis", type=int, default=-1, metavar="DEVICE_VECTORS",
-                    help="also solve with a hybrid basis of this many device columns "
-                         "(the rest in pinned host memory) and report its cost")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "single-GPU build: only --gpus 1 is supported; "
-                                   "the multi-GPU sharding is described, not built"}))
-        sys.exit(2)
-
-    _load_package()
-    if args.copy_calibration:
-        return copy_calibration()
-    if args.filter_only:
-        return filter_only(args)
-    if args.reorth_only:
-        return reorth_only(args)
-    if args.packed_compare:
-        return packed_compare(args)
-    if args.workload == "qsun-batch":
-        return qsun_batch(args)
-    if args.workload == "xxz-l22":
-        return xxz_l22(args)
-    if args.workload == "j1j2-l24":
-        return j1j2_l24(args)
-    if args.workload == "xxz-l14":
-        return xxz_l14(args)
+def plain_bench(args):
+    """``--workload xxz-l20``: one complete FP64 solve of XXZ L=20 (D=184756,
+    W=3 seed 1, block 8, nev=100) per step, with FP32-filter forms beside."""
     import numpy as np
     import torch
     from polfed_amd.device import Device
@@ -772,7 +941,8 @@ def main():
     ms_step, bytes_step, nsteps, filter_ms = timer.summary(op.D, 8)
     rep = reports[-1]
     achieved = bytes_step / (ms_step * 1e-3) / 1e9
-    kname, traffic = measured_traffic(kernel_name(op))
+    kname, klayout = kernel_name(op)
+    traffic = measured_traffic(kname)
     out = {
         "metric": METRIC,
         "value": elapsed / args.steps,
@@ -793,7 +963,7 @@ def main():
                   "Emin": rep.Emin, "Emax": rep.Emax, "times_s": rep.times},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "kernel": kname,
+                     "traffic": traffic, "kernel": kname, "layout": klayout,
                      "per_launch_bytes": bytes_step, "avg_launch_ms": ms_step,
                      "filter_share": filter_ms * 1e-3 / elapsed},
     }
@@ -847,6 +1017,74 @@ def main():
         q = dev.to_host(x0)
         out["cpu_baseline"] = cpu_baseline(op, rep, q)
     print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="skip the FP32-filter solves (mixed and pure FP32)")
+    ap.add_argument("--filter-only", type=int, default=0, metavar="STEPS",
+                    help="profiling driver: time about STEPS filter steps only")
+    ap.add_argument("--dtype", default="f64", choices=["f64", "f32"],
+                    help="filter arithmetic for --filter-only")
+    ap.add_argument("--packed-compare", action="store_true",
+                    help="profiling driver: packed sparse operator vs matrix-free filter steps")
+    ap.add_argument("--layout", default="", metavar="V,PBLOCKS,WIDE,PARTITION",
+                    help="--filter-only: fixed step-kernel launch configuration instead of "
+                         "the autotuner's (e.g. 2,8,0,1)")
+    ap.add_argument("--row-partition", type=int, default=-1, choices=[-1, 0, 1],
+                    help="--filter-only: spin-chain row order (-1 autotuned, 0 reference, "
+                         "1 XCD partition)")
+    ap.add_argument("--reorth-only", type=int, default=0, metavar="CALLS",
+                    help="profiling driver: time CALLS CGS2 reorthogonalizations")
+    ap.add_argument("--workload", default="xxz-l22",
+                    choices=["xxz-l22", "xxz-l20", "xxz-l14", "qsun-batch", "j1j2-l24"],
+                    help="xxz-l22: the bench line (FP64 and FP32-filter solves, FP32 filter "
+                         "steps); xxz-l20: the plain FP64 solve as the step; xxz-l14: the "
+                         "CPU-runnable case against the oracle's full CPU solve; qsun-batch: 64 "
+                         "Quantum-Sun instances; j1j2-l24: KPM and filter at L=24")
+    ap.add_argument("--instances", type=int, default=64, help="qsun-batch: number of instances")
+    ap.add_argument("--modes", default="mixed,f64",
+                    help="qsun-batch: comma list of mixed, f64 (batched) and plain (FP64, "
+                         "one polfed call per instance)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="profiling driver: one 256 MiB device copy, five times")
+    ap.add_argument("--switch-tol", type=float, default=None,
+                    help="mixed-precision solves: residual estimate at which the filter "
+                         "switches to FP32 (default: TransformConfig's)")
+    ap.add_argument("--modes-l20", default="mixed,f32",
+                    help="FP32-filter solves reported beside the FP64 bench solve")
+    ap.add_argument("--hybrid-basis", type=int, default=-1, metavar="DEVICE_VECTORS",
+                    help="also solve with a hybrid basis of this many device columns "
+                         "(the rest in pinned host memory) and report its cost")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "single-GPU build: only --gpus 1 is supported; "
+                                   "the multi-GPU sharding is described, not built"}))
+        sys.exit(2)
+
+    _load_package()
+    if args.copy_calibration:
+        return copy_calibration()
+    if args.filter_only:
+        return filter_only(args)
+    if args.reorth_only:
+        return reorth_only(args)
+    if args.packed_compare:
+        return packed_compare(args)
+    if args.workload == "qsun-batch":
+        return qsun_batch(args)
+    if args.workload == "xxz-l22":
+        return precision_bench(args)
+    if args.workload == "j1j2-l24":
+        return j1j2_l24(args)
+    if args.workload == "xxz-l14":
+        return xxz_l14(args)
+    return plain_bench(args)
 
 
 if __name__ == "__main__":

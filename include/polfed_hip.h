@@ -233,6 +233,14 @@ int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value);
 int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nsteps, double* best_us,
                 int32_t* best_vec, int32_t* best_xcd, pf_stream stream);
 
+/* The launch configuration the filter and KPM drivers use for (dtype, ncols)
+ * on this operator -- the autotuned one with any pf_op_set_tuning overrides --
+ * as cfg[8] = {vector width V, XCD row mapping, non-temporal hints, persistent
+ * blocks per CU, wide gather batches, grain shuffles, 8-waves-per-SIMD build,
+ * partition row order}.  No counterpart in the reference (an observability
+ * helper for profiling the exact kernel instance a solve runs). */
+int pf_op_get_tuning(pf_op* op, int32_t dtype, int32_t ncols, int32_t* cfg);
+
 /* ------------------------------------------------------------------- KPM */
 
 /* Fill R probe columns (device, column-major, ld) with a counter-based

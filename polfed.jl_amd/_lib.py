@@ -99,6 +99,7 @@ SIGNATURES = {
                                      _vp]),
     "pf_op_set_tuning": (_c.c_int, [_vp, _i32, _i32]),
     "pf_autotune": (_c.c_int, [_vp, _i32, _i32, _i32, _pdbl, _pi32, _pi32, _vp]),
+    "pf_op_get_tuning": (_c.c_int, [_vp, _i32, _i32, _pi32]),
     "pf_fill_probes": (_c.c_int, [_i32, _u64, _i64, _i32, _vp, _i64, _vp]),
     "pf_kpm_moments": (_c.c_int, [_vp, _i32, _i32, _i32, _vp, _i64, _pdbl, _vp]),
     "pf_gram": (_c.c_int, [_vp, _vp, _i64, _i32, _vp, _i64, _i32, _i64, _vp, _i32, _vp]),

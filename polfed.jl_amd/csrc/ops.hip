@@ -1447,6 +1447,18 @@ extern "C" int pf_op_set_tuning(pf_op* op, int32_t key, int32_t value) {
 
 // Times the fused recurrence step for every candidate (vector width, XCD
 // mapping) on the operator's own shape and keeps the fastest for (dtype, ncols).
+extern "C" int pf_op_get_tuning(pf_op* op, int32_t dtype, int32_t ncols, int32_t* cfg) {
+  return guarded([&] {
+    PF_REQUIRE(op && cfg, "get_tuning: null argument");
+    PF_REQUIRE(dtype == PF_F64 || dtype == PF_F32, "get_tuning: dtype");
+    PF_REQUIRE(ncols >= 1 && ncols <= chunk_cols(dtype), "get_tuning: 1 <= ncols <= chunk width");
+    const Layout lay = tuned_layout(op, dtype, ncols);
+    const int32_t v[8] = {lay.V, lay.xcd, lay.nt, lay.pblocks, lay.wide, lay.shfl, lay.occ,
+                          lay.part};
+    for (int i = 0; i < 8; ++i) cfg[i] = v[i];
+  });
+}
+
 extern "C" int pf_autotune(pf_op* op, int32_t dtype, int32_t ncols, int32_t nsteps,
                            double* best_us, int32_t* best_vec, int32_t* best_xcd,
                            pf_stream stream) {

@@ -166,6 +166,16 @@ class Operator:
                                    ctypes.byref(vec), ctypes.byref(xcd), self.device.stream))
         return us.value, vec.value, xcd.value
 
+    def tuning(self, ncols: int, dtype="f64") -> dict:
+        """The launch configuration the drivers use for (dtype, ncols): the
+        autotuned one with any set_tuning overrides (pf_op_get_tuning)."""
+        code, _ = _dtype(dtype)
+        cfg = (ctypes.c_int32 * 8)()
+        check(self.lib.pf_op_get_tuning(self.op, code, int(ncols), cfg))
+        keys = ("vec", "xcd", "nt", "pblocks", "wide", "grain_shuffle", "occupancy",
+                "row_partition")
+        return dict(zip(keys, (int(v) for v in cfg)))
+
     def ensure_tuned(self, ncols: int, dtype="f64", nsteps: int = 30):
         """Autotune (dtype, ncols) once per operator."""
         done = self.__dict__.setdefault("_tuned", {})

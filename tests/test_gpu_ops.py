@@ -132,6 +132,8 @@ def test_filter_persistent_grid_settings_agree(dev, pblocks, wide, vec, occ):
     c = oc.filter_coefficients(-0.2, 150)
     X = np.random.default_rng(7).standard_normal((H.shape[0], 8))
     op.set_tuning(pblocks=pblocks, wide=wide, vec=vec, occupancy=occ)
+    t = op.tuning(8, "f64")  # the configuration the driver launches (pf_op_get_tuning)
+    assert (t["pblocks"], t["wide"], t["vec"], t["occupancy"]) == (pblocks, wide, vec, occ)
     Y = dev.to_host(op.filter(c, dev.to_device(X)))
     assert _rel(Y, oc.clenshaw_apply(lambda Z: Ht @ Z, c, X)) < 1e-12
 

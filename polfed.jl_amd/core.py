@@ -470,7 +470,10 @@ def polfed_batch(ops, x0s, howmany: int, target, *, produce_report: bool = False
     batch.set_filter([p.coeffs for p in preps], polytypes.pop())
     tuned = None
     if autotune:
-        tuned = batch.autotune("f64" if transform.dtype == "f64" else "f32")
+        # one tiling serves both arithmetics: tune for the one that carries
+        # most of the work (FP64 in the mixed form, whose FP32 iterations come
+        # last)
+        tuned = batch.autotune("f32" if transform.dtype == "f32" else "f64")
     setup_s = time.perf_counter() - t_all
 
     fast_mode = {"f64": "never", "f32": "always", "mixed": "mixed"}[transform.dtype]

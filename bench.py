@@ -454,6 +454,16 @@ def precision_workload(dev):
     return op, dev.to_device(q), q
 
 
+def _stage_times(rep):
+    """Seconds per stage of a solve: bounds probes, DoS, order, and the
+    Lanczos stages (filter = mapping; the host projected eigensolver sits in
+    convergence)."""
+    keys = ("bounds", "dos", "transform", "lanczos.mapping", "lanczos.tridiagonalization",
+            "lanczos.orthogonalization", "lanczos.reorthogonalization", "lanczos.convergence",
+            "lanczos.switch_test", "lanczos.polish")
+    return {k.replace("lanczos.", ""): round(rep.times[k], 3) for k in keys if k in rep.times}
+
+
 def _pair_errors(vals, res, ref_vals, width):
     """Each returned pair's eigenvalue error against the nearest FP64-run
     eigenvalue (over the spectral width) and its residual."""
@@ -613,7 +623,8 @@ def precision_bench(args):
         "workload": "XXZ L=22 Sz=0 D=705432 W=2 seed 2, block 16, nev=200, target middle, "
                     "eigentol 1e-10, same start block and KPM probes for both solves",
         "fp64": {"time_s": t64, "order": rep64.order, "iterations": rep64.iterations,
-                 "pairs": int(len(v64)), "max_residual": float(r64.max()) if r64.size else None},
+                 "pairs": int(len(v64)), "max_residual": float(r64.max()) if r64.size else None,
+                 "stage_s": _stage_times(rep64)},
         "fp32_filter": {"time_s": t32, "mode": f"mixed (switch_tol {sw:g}) + FP64 polish",
                         "order": rep32.order, "iterations": rep32.iterations,
                         "fp32_iterations": int(lz32.fast_iterations),
@@ -621,7 +632,8 @@ def precision_bench(args):
                         "fp64_fallback": bool(lz32.fallback),
                         "polish_s": float(lz32.times.get("polish", 0.0)),
                         "pairs": int(len(v32)), "pairs_residual_le_1e-10": ok32,
-                        "max_residual": float(res32.max()) if res32.size else None},
+                        "max_residual": float(res32.max()) if res32.size else None,
+                        "stage_s": _stage_times(rep32)},
         "speedup_fp32_filter_vs_fp64": t64 / t32,
         "per_pair": {"eig_err_over_width": [float(f"{e:.2g}") for e in eerr],
                      "residual": [float(f"{r:.2g}") for r in res32]},
@@ -670,7 +682,8 @@ def precision_bench(args):
             "time_to_nev": time_to_nev,
             "kpm_fp32_vs_fp64": {"moments": 250, "probes": 300,
                                  "max_err_over_D": float(kerr.max()),
-                                 "median_err_over_D": float(np.median(kerr))},
+                                 "median_err_over_D": float(np.median(kerr)),
+                                 "per_moment_err_over_D": [float(f"{e:.1g}") for e in kerr]},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                          "traffic": traffic, "kernel": kname, "layout": klayout,

@@ -243,9 +243,9 @@ function LZ.reorthogonalization!(B::ROCBasis, W::ROCMatrix{Float64}, ::LZ.Proces
 end
 
 # ------------------------------------------------------- backend registry
-_memory_info() = AMDGPU.info()
-_available_memory() = first(AMDGPU.info())
-_total_memory() = last(AMDGPU.info())
+_memory_info() = AMDGPU.Mem.info()              # (free, total) bytes of the device
+_available_memory() = first(AMDGPU.Mem.info())
+_total_memory() = last(AMDGPU.Mem.info())
 
 function __init__()
     Polfed._set_cuda_backend!(

@@ -234,3 +234,29 @@ def test_polfed_rejects_blocks_wider_than_64(dev):
     x0 = np.random.default_rng(4).standard_normal((op.D, 65))
     with pytest.raises(ValueError, match="at most 64 columns"):
         polfed(op, x0, 70, "middle")
+
+
+def test_projected_eigensolver_on_the_device(dev):
+    """From k = DEVICE_EIGH_MIN the projected eigenproblem runs on the device:
+    same eigenvalues as the host banded solver, orthonormal eigenvectors with
+    small residuals, and the index subset of the switch test."""
+    from polfed_amd.lanczos import DEVICE_EIGH_MIN, _proj_eigh
+    k, s = DEVICE_EIGH_MIN + 96, 16
+    rng = np.random.default_rng(11)
+    T = np.zeros((k, k))
+    for i in range(0, k, s):
+        A = rng.standard_normal((s, s))
+        T[i:i + s, i:i + s] = A + A.T
+        if i + s < k:
+            R = np.triu(rng.standard_normal((s, s)))
+            T[i:i + s, i + s:i + 2 * s] = R.T
+            T[i + s:i + 2 * s, i:i + s] = R
+    lam_h, _ = _proj_eigh(T, s)
+    lam_d, phi_d = _proj_eigh(T, s, device=dev.torch_device)
+    scale = np.max(np.abs(lam_h))
+    assert np.max(np.abs(lam_d - lam_h)) <= 1e-12 * scale
+    assert np.linalg.norm(phi_d.T @ phi_d - np.eye(k)) <= 1e-11
+    assert np.max(np.linalg.norm(T @ phi_d - phi_d * lam_d, axis=0)) <= 1e-11 * scale
+    ls, ps = _proj_eigh(T, s, [k - 20, k - 1], device=dev.torch_device)
+    assert np.array_equal(ls, lam_d[k - 20:])
+    assert np.array_equal(ps, phi_d[:, k - 20:])

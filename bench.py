@@ -894,8 +894,13 @@ def copy_calibration():
 
 
 def _switch(args):
-    """TransformConfig keywords for a --switch-tol override."""
-    return {} if getattr(args, "switch_tol", None) is None else {"switch_tol": args.switch_tol}
+    """TransformConfig keywords for --switch-tol / --polish-fraction overrides."""
+    kw = {}
+    if getattr(args, "switch_tol", None) is not None:
+        kw["switch_tol"] = args.switch_tol
+    if getattr(args, "polish_fraction", None) is not None:
+        kw["polish_fraction"] = args.polish_fraction
+    return kw
 
 
 def solve(op, x0, fp="f64", basistype=None, switch_tol=None):
@@ -1098,6 +1103,9 @@ def main():
     ap.add_argument("--switch-tol", type=float, default=None,
                     help="mixed-precision solves: residual estimate at which the filter "
                          "switches to FP32 (default: TransformConfig's)")
+    ap.add_argument("--polish-fraction", type=float, default=None,
+                    help="FP32-filter solves: degree of the FP64 polish window as a fraction "
+                         "of the filter order (default: TransformConfig's)")
     ap.add_argument("--modes-l20", default="mixed,f32",
                     help="FP32-filter solves reported beside the FP64 bench solve")
     ap.add_argument("--hybrid-basis", type=int, default=-1, metavar="DEVICE_VECTORS",

@@ -128,7 +128,7 @@ class TransformConfig:
     dtype: str = "f64"
     switch_tol: float = 1e-4
     polish: bool = True
-    polish_fraction: float = 1.0 / 16.0
+    polish_fraction: float = 1.0 / 32.0
 
 
 class MatrixBasis:

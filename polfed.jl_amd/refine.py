@@ -34,7 +34,7 @@ def window_coefficients(target: float, degree: int) -> np.ndarray:
     return c / float(spc.series_value(c, target))
 
 
-def polish_degree(order: int, fraction: float = 1.0 / 16.0, minimum: int = 32) -> int:
+def polish_degree(order: int, fraction: float = 1.0 / 32.0, minimum: int = 32) -> int:
     """Degree of the polish window: a fraction of the main filter's order."""
     return max(minimum, int(order * fraction))
 
@@ -60,7 +60,7 @@ def _orthonormalize(dev, Z: torch.Tensor) -> torch.Tensor:
 
 
 def polish(dev, op, Y: torch.Tensor, keep: int, *, target: float, order: int, coeffs: np.ndarray,
-           polytype: str, which: str, fraction: float = 1.0 / 16.0):
+           polytype: str, which: str, fraction: float = 1.0 / 32.0):
     """Polished pairs from the Ritz block Y ((n, D) device tensor, n >= keep:
     converged Ritz vectors only, see lanczos.lanczos).
 

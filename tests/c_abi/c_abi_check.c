@@ -4,6 +4,8 @@
  *   - pf_apply on unit vectors against the pf_op_connectivity export,
  *   - pf_filter with coefficients (0, 0, 1) against 2 H~(H~ x) - x built from
  *     two rescaled pf_apply calls (T_2(x) = 2x^2 - 1),
+ *   - pf_filter_noalloc giving pf_filter's output bit for bit, and
+ *     pf_op_get_tuning reporting the launch configuration,
  *   - an invalid operator request returning an error code and a message.
  * Prints "c_abi_check: ok ..." and exits 0 on success. */
 #include <math.h>
@@ -105,6 +107,15 @@ int main(void) {
   CHECK(pf_filter(op, PF_F64, PF_CHEBYSHEV, coeffs, 2, dx, D, dy, D, 3, NULL));
   double* hf = malloc(D * 3 * sizeof(double));
   HIPCHECK(hipMemcpy(hf, dy, D * 3 * sizeof(double), hipMemcpyDeviceToHost));
+  /* the same filter without allocation: identical bits */
+  CHECK(pf_filter_noalloc(op, PF_F64, PF_CHEBYSHEV, coeffs, 2, dx, D, dz, D, 3, NULL));
+  double* hn = malloc(D * 3 * sizeof(double));
+  HIPCHECK(hipMemcpy(hn, dz, D * 3 * sizeof(double), hipMemcpyDeviceToHost));
+  const int noalloc_ok = memcmp(hn, hf, D * 3 * sizeof(double)) == 0;
+  free(hn);
+  int32_t cfg[8];
+  CHECK(pf_op_get_tuning(op, PF_F64, 3, cfg));
+  const int tuning_ok = cfg[0] >= 1 && cfg[0] <= 32 && cfg[3] >= 0;
   CHECK(pf_apply(op, PF_F64, 1, dx, D, dz, D, 3, NULL));
   CHECK(pf_apply(op, PF_F64, 1, dz, D, dy, D, 3, NULL));
   HIPCHECK(hipMemcpy(hy, dy, D * 3 * sizeof(double), hipMemcpyDeviceToHost));
@@ -138,9 +149,9 @@ int main(void) {
   free(dg);
   CHECK(pf_op_destroy(op));
   CHECK(pf_destroy(H));
-  if (err_apply > 1e-14 || err_filter > 1e-14 || !err_ok) {
-    fprintf(stderr, "c_abi_check: FAILED apply %.3e filter %.3e error-path %d\n", err_apply,
-            err_filter, err_ok);
+  if (err_apply > 1e-14 || err_filter > 1e-14 || !err_ok || !noalloc_ok || !tuning_ok) {
+    fprintf(stderr, "c_abi_check: FAILED apply %.3e filter %.3e error-path %d noalloc %d "
+            "tuning %d\n", err_apply, err_filter, err_ok, noalloc_ok, tuning_ok);
     return 1;
   }
   printf("c_abi_check: ok D=%lld apply err %.1e filter err %.1e error path \"%s\"\n",

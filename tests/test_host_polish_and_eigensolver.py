@@ -26,7 +26,7 @@ def test_window_coefficients(target, degree):
 
 def test_polish_degree():
     from polfed_amd import refine
-    assert refine.polish_degree(28000) == 1750
+    assert refine.polish_degree(28000) == 875
     assert refine.polish_degree(100) == 32
     assert refine.polish_degree(28000, 0.25) == 7000
 

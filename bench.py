@@ -245,7 +245,7 @@ def batch_filter_only(args, dev):
     rows = sum(op.D for op in ops)
     esz = 8 if dt == "f64" else 4
     print(json.dumps({"mode": "batch-filter-only", "dtype": dt, "instances": len(ops),
-                      "rows": rows, "autotune_us_per_step": tuned_us,
+                      "rows": rows, "tuning": b.tuning(), "autotune_us_per_step": tuned_us,
                       "stream_us_per_step": us, "steps": reps * (K + 1),
                       "algorithmic_GBps": 4 * rows * esz / (us * 1e-6) / 1e9}), flush=True)
 

@@ -342,13 +342,27 @@ enum {
   PF_BATCH_VEC = 3,            /* columns per lane: 1/2/4/8 dividing ncols        */
   PF_BATCH_ROWS_PER_LANE = 4,  /* rows per lane: 1..64                            */
   PF_BATCH_XCD = 5,            /* 1: XCD-contiguous block mapping (default)       */
-  PF_BATCH_GRAIN_SHUFFLE = 6   /* 1: grain block by cross-lane shuffles (default) */
+  PF_BATCH_GRAIN_SHUFFLE = 6,  /* 1: grain block by cross-lane shuffles (default) */
+  PF_BATCH_LDS_ROWS = 7        /* 1: a block stages its rows in LDS and reads the
+                                  grain block and the flips that stay among them
+                                  from there; used when ncols is a power of two,
+                                  ncols / vec <= 16, rows per lane a power of two
+                                  and a multiple of 4 / vec (vec < 4), and the
+                                  block's rows (256 / (ncols / vec) x rows per
+                                  lane) hold <= 32 KiB in FP64; ignored otherwise
+                                  (default 0)                                     */
 };
 /* Launch settings of the batch.  No reference counterpart. */
 int pf_batch_set_tuning(pf_batch* b, int32_t key, int32_t value);
 
+/* The batch's launch settings: cfg[0..7] = columns per lane, rows per lane,
+ * XCD mapping, grain shuffles, LDS rows requested, LDS rows in use (the
+ * request and the tiling allow it), graph replays, step nodes per graph.
+ * No reference counterpart. */
+int pf_batch_get_tuning(const pf_batch* b, int32_t* cfg);
+
 /* Times every (columns per lane, rows per lane, XCD mapping, grain-block
- * shuffles or reloads) tiling of the multi-instance step kernel on all
+ * shuffles or reloads, rows in LDS or not) tiling of the multi-instance step kernel on all
  * instances in arithmetic `dtype`, then graph replays against stream
  * launches, and keeps the fastest.  best_us
  * (host, may be NULL): time per step of the whole batch.  Overwrites the

@@ -42,6 +42,7 @@ PF_BATCH_VEC = 3
 PF_BATCH_ROWS_PER_LANE = 4
 PF_BATCH_XCD = 5
 PF_BATCH_GRAIN_SHUFFLE = 6
+PF_BATCH_LDS_ROWS = 7
 
 PF_PROBE_GAUSSIAN = 0
 PF_PROBE_PHASE = 1
@@ -116,6 +117,7 @@ SIGNATURES = {
     "pf_batch_filter": (_c.c_int, [_vp, _pi32, _pi32, _c.POINTER(_vp), _c.POINTER(_i64),
                                    _c.POINTER(_vp), _c.POINTER(_i64), _vp]),
     "pf_batch_set_tuning": (_c.c_int, [_vp, _i32, _i32]),
+    "pf_batch_get_tuning": (_c.c_int, [_vp, _pi32]),
     "pf_batch_autotune": (_c.c_int, [_vp, _i32, _i32, _pdbl, _vp]),
 }
 

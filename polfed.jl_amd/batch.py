@@ -23,7 +23,8 @@ from .operators import QSunOperator, _dtype
 
 _TUNE = {"graph": _lib.PF_BATCH_GRAPH, "graph_steps": _lib.PF_BATCH_GRAPH_STEPS,
          "vec": _lib.PF_BATCH_VEC, "rows_per_lane": _lib.PF_BATCH_ROWS_PER_LANE,
-         "xcd": _lib.PF_BATCH_XCD, "grain_shuffle": _lib.PF_BATCH_GRAIN_SHUFFLE}
+         "xcd": _lib.PF_BATCH_XCD, "grain_shuffle": _lib.PF_BATCH_GRAIN_SHUFFLE,
+         "lds_rows": _lib.PF_BATCH_LDS_ROWS}
 
 
 class QSunBatch:
@@ -108,11 +109,21 @@ class QSunBatch:
                                        i64(ldy), self.device.stream))
 
     def set_tuning(self, **kw) -> None:
-        """graph (0/1), graph_steps, vec, rows_per_lane, xcd, grain_shuffle (0/1)."""
+        """graph (0/1), graph_steps, vec, rows_per_lane, xcd, grain_shuffle (0/1),
+        lds_rows (0/1)."""
         for k, v in kw.items():
             if k not in _TUNE:
                 raise ValueError(f"unknown batch tuning key {k!r}")
             check(self.lib.pf_batch_set_tuning(self.b, _TUNE[k], int(v)))
+
+    def tuning(self) -> dict:
+        """The launch settings in force (pf_batch_get_tuning); lds_used says
+        whether the LDS form runs (requested and allowed by the tiling)."""
+        cfg = (ctypes.c_int32 * 8)()
+        check(self.lib.pf_batch_get_tuning(self.b, cfg))
+        keys = ("vec", "rows_per_lane", "xcd", "grain_shuffle", "lds_rows", "lds_used",
+                "graph", "graph_steps")
+        return dict(zip(keys, list(cfg)))
 
     def autotune(self, dtype="f32", nsteps: int = 40) -> float:
         """Pick the fastest tiling and graph/stream mode; returns the time per

@@ -71,7 +71,11 @@ struct BatchArgs {
   int rpt;   // block passes (rows per lane)
   int xcd;   // XCD-contiguous block mapping
   int shfl;  // 1: grain block by cross-lane shuffles, 0: by reloads
+  int lgrows;  // LDS form: log2(rows per block)
 };
+
+// LDS form: the block's rows of b2 (dynamic shared memory, rows x row stride)
+extern __shared__ __align__(16) char batch_rows[];
 
 __device__ __forceinline__ void poly_ab_dev(int type, int k, double& alpha, double& beta_k) {
   switch (type) {
@@ -190,8 +194,138 @@ __device__ __forceinline__ bool batch_locate(const BatchArgs& A, int kexpl, int 
   return !(k < 0 || dy.K < 0 || k > dy.K);  // uniform over the block
 }
 
-// One Clenshaw step of every active instance whose arithmetic is T.
+// batch_body with the block's rows of b2 staged in LDS: the block owns the
+// aligned range [r0, r0 + 2^lgrows) of its instance (G a power of two, so
+// every thread has a row), copies it to LDS, and reads the grain block and
+// the flips that stay in the range from there (qsun_hrow_lds).
 template <typename T, int V>
+__device__ __forceinline__ void batch_body_lds(const BatchArgs& A, const BatchInst& I,
+                                               const BatchDyn& dy, int k, int64_t lb) {
+  typedef typename VT<T, V>::t vt;
+  __shared__ T sG[kMaxDg * kMaxDg];
+  __shared__ T sC[kMaxLocPad];
+  __shared__ uint32_t sF[kMaxLocPad];
+  const int dg = 1 << I.Lg;
+  const uint32_t rsb = (uint32_t)(A.s * sizeof(T));
+  const int rsh = __ffs(rsb) - 1;  // the row stride is a power of two in this form
+  const int nl = qsun_local_flips(A.lgrows, I.Lg, I.Lloc);
+  constexpr int B = kLdsFlipBatch;
+  const int nlp = (nl + B - 1) / B * B, nrp = (I.Lloc - nl + B - 1) / B * B;
+  qsun_stage_split<T>(I.Lloc, nl, dg, I.flip, I.coup, I.G, I.scale, rsh, sF, sC, sG);
+
+  const int g = dy.K - k;  // local step, >= 0
+  const bool hasH = g >= 1;
+  double cxd, chd, cbd;
+  const double ck = A.coefs ? A.coefs[I.coef_off + k] : 0.5;
+  if (k >= 1) {
+    double alpha, beta1, dummy;
+    poly_ab_dev(A.polytype, k, alpha, dummy);
+    poly_ab_dev(A.polytype, k + 1, dummy, beta1);
+    cxd = ck;
+    chd = alpha;
+    cbd = beta1;
+  } else {
+    double dummy, beta1;
+    poly_ab_dev(A.polytype, 1, dummy, beta1);
+    cxd = ck;  // phi0 = 1 for every family
+    chd = A.polytype == PF_HERMITE ? 2.0 : 1.0;
+    cbd = beta1;
+  }
+  const bool hasB3 = g >= 2 && cbd != 0.0;
+  const T cx = (T)cxd, ch = (T)chd, cb = (T)cbd;
+
+  const int64_t off = I.row0 * (int64_t)A.s * 8;  // byte offset of this instance's region
+  char* bufs[3] = {A.B0, A.B1, A.B2};
+  const T* __restrict__ X = reinterpret_cast<const T*>(A.Xb + off);
+  T* __restrict__ OUT = reinterpret_cast<T*>(bufs[rot_slot(g)] + off);
+  const char* B2 = bufs[rot_slot(g + 2)] + off;                                       // g-1
+  const T* __restrict__ B3 = reinterpret_cast<const T*>(bufs[rot_slot(g + 1)] + off);  // g-2
+
+  const int rloc = threadIdx.x / A.G;
+  const int gl = threadIdx.x - rloc * A.G;
+  const uint32_t cofs_b = (uint32_t)(gl * V * sizeof(T));
+  const int64_t r0 = lb * A.rpt * A.rpb;  // first row of the block, a multiple of 2^lgrows
+  // A block larger than its instance (D < 2^lgrows, so r0 = 0) reads its
+  // rows >= D at row mod D (D is a power of two) and stores nothing for them:
+  // every row group is whole and needs no branch.
+  const uint32_t dmask = (uint32_t)(I.D - 1);
+  constexpr int R = qsun_lds_rows<T, V>();  // A.rpt is a multiple of R
+  typedef __attribute__((address_space(3))) vt lvt;
+  typedef __attribute__((address_space(3))) char lch;
+  if (hasH) {  // uniform over the block
+    for (int u0 = 0; u0 < A.rpt; u0 += R) {
+      vt xr[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        const uint32_t row = (uint32_t)(r0 + (u0 + i) * A.rpb + rloc) & dmask;
+        xr[i] = gload<T, V>(B2, (row << rsh) + cofs_b);
+      }
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+        *reinterpret_cast<lvt*>((lch*)batch_rows +
+                                ((uint32_t)((u0 + i) * A.rpb + rloc) << rsh) + cofs_b) = xr[i];
+    }
+  }
+  __syncthreads();  // the LDS tables and rows
+
+  QsunRow q;
+  q.b2 = B2;
+  q.cofs_b = cofs_b;
+  q.rsb = rsb;
+  q.rsh = rsh;
+  q.dg = dg;
+  q.nloc = nlp + nrp;
+  q.lb = I.lb;
+  q.dlo = I.qdiag;
+  q.dhi = I.qdiag + ((int64_t)1 << I.lb);
+  q.scale = I.scale;
+  q.shift = I.shift;
+  q.sF = sF;
+  q.sC = sC;
+  const int64_t cofs = (int64_t)gl * V;
+  for (int u0 = 0; u0 < A.rpt; u0 += R) {
+    uint32_t row[R], rl_b[R];
+    vt xv[R], b3v[R], hv[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int rl = (u0 + i) * A.rpb + rloc;
+      row[i] = (uint32_t)(r0 + rl) & dmask;
+      rl_b[i] = ((uint32_t)rl << rsh) + cofs_b;
+      const int64_t o = (int64_t)row[i] * A.s + cofs;
+      xv[i] = *as_global(reinterpret_cast<const vt*>(X + o));
+      b3v[i] = hasB3 ? *as_global(reinterpret_cast<const vt*>(B3 + o)) : (vt)(T)0;
+      hv[i] = (vt)(T)0;
+    }
+    if (hasH) qsun_hrows_lds_dg<T, V, R>(q, row, rl_b, batch_rows, sG, nlp, nrp, hv);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if (r0 + (u0 + i) * A.rpb + rloc >= I.D) continue;
+      vt out = cx * xv[i];
+      if (hasH) out += ch * hv[i];
+      if (hasB3) out += cb * b3v[i];
+      const int64_t o = (int64_t)row[i] * A.s + cofs;
+      if (k >= 1) {
+        *as_global(reinterpret_cast<vt*>(OUT + o)) = out;
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          as_global(dy.Y)[(int64_t)row[i] + (int64_t)(gl * V + v) * dy.ldy] =
+              (double)vget<T, V>(out, v);
+      }
+    }
+  }
+}
+
+template <typename T, int V, bool LDS>
+__device__ __forceinline__ void batch_body_any(const BatchArgs& A, const BatchInst& I,
+                                               const BatchDyn& dy, int k, int64_t lb) {
+  if constexpr (LDS) batch_body_lds<T, V>(A, I, dy, k, lb);
+  else batch_body<T, V>(A, I, dy, k, lb);
+}
+
+// One Clenshaw step of every active instance whose arithmetic is T (LDS: the
+// block rows staged in LDS).
+template <typename T, int V, bool LDS>
 __global__ __launch_bounds__(256) void qsun_batch_step_kernel(const BatchArgs A, int kexpl,
                                                               int j) {
   int lo, k;
@@ -199,25 +333,28 @@ __global__ __launch_bounds__(256) void qsun_batch_step_kernel(const BatchArgs A,
   int64_t lb;
   constexpr int kDt = sizeof(T) == 8 ? PF_F64 : PF_F32;
   if (!batch_locate(A, kexpl, j, lo, dy, k, lb) || dy.dtype != kDt) return;
-  batch_body<T, V>(A, A.insts[lo], dy, k, lb);
+  batch_body_any<T, V, LDS>(A, A.insts[lo], dy, k, lb);
 }
 
 // One Clenshaw step of every active instance, FP64 and FP32 instances in the
 // same launch (each block takes its instance's arithmetic): an application
 // with both kinds costs one launch per step, not two over the whole grid.
-template <int V>
+template <int V, bool LDS>
 __global__ __launch_bounds__(256) void qsun_batch_step_kernel_mixed(const BatchArgs A, int kexpl,
                                                                     int j) {
   int lo, k;
   BatchDyn dy;
   int64_t lb;
   if (!batch_locate(A, kexpl, j, lo, dy, k, lb)) return;
-  if (dy.dtype == PF_F64) batch_body<double, V>(A, A.insts[lo], dy, k, lb);
-  else batch_body<float, V>(A, A.insts[lo], dy, k, lb);
+  if (dy.dtype == PF_F64) batch_body_any<double, V, LDS>(A, A.insts[lo], dy, k, lb);
+  else batch_body_any<float, V, LDS>(A, A.insts[lo], dy, k, lb);
 }
 
 // dtype bit set of an application: bit 0 FP64 instances, bit 1 FP32 ones
 constexpr int kMaskF64 = 1, kMaskF32 = 2;
+
+// LDS form: most bytes of b2 rows a block stages (FP64 rows)
+constexpr size_t kBatchLdsBytes = 32 << 10;
 
 __global__ void batch_kset_kernel(int* kstate, int value) {
   if (threadIdx.x == 0 && blockIdx.x == 0) kstate[0] = value;
@@ -249,7 +386,7 @@ struct pf_batch {
   bool have_filter = false;
   DevBuf insts, dyn, blk0d, coefs, kstate, bufs;
   // tuning
-  int V = 1, rpt = 1, xcd = 1, use_graph = 1, gsteps = 96, shfl = 1;
+  int V = 1, rpt = 1, xcd = 1, use_graph = 1, gsteps = 96, shfl = 1, lds = 0;
   // pinned staging of the per-application records; the event marks the end
   // of the last copy out of it
   BatchDyn* hdyn = nullptr;
@@ -262,6 +399,27 @@ struct pf_batch {
 
   int G() const { return s / V; }
   int rpb() const { return 256 / G(); }
+  // The LDS form needs power-of-two rows per block and row strides, at least
+  // kMaxDg rows per pass (the rows of a lane then share their grain
+  // position), rows per lane in whole FP32 row groups, and the block's rows
+  // within kBatchLdsBytes in FP64.
+  bool lds_ok() const {
+    const int rows = rpb() * rpt;
+    return 256 % G() == 0 && (rows & (rows - 1)) == 0 && rpb() >= kMaxDg &&
+           rpt % lds_rows_f32() == 0 && (s & (s - 1)) == 0 &&
+           (size_t)rows * s * 8 <= kBatchLdsBytes;
+  }
+  bool use_lds() const { return lds && lds_ok(); }
+  // rows a lane takes at once in the LDS form with FP32 (a multiple of the
+  // FP64 figure)
+  int lds_rows_f32() const {
+    switch (V) {
+      case 1: return qsun_lds_rows<float, 1>();
+      case 2: return qsun_lds_rows<float, 2>();
+      case 4: return qsun_lds_rows<float, 4>();
+      default: return qsun_lds_rows<float, 8>();
+    }
+  }
   void drop_graph() {
     for (int m = 0; m < 4; ++m) {
       if (exec[m]) (void)hipGraphExecDestroy(exec[m]);
@@ -350,13 +508,29 @@ struct pf_batch {
     A.rpt = rpt;
     A.xcd = xcd;
     A.shfl = shfl;
+    A.lgrows = __builtin_ctz((unsigned)(rpb() * rpt));
     return A;
   }
-  template <typename T>
+  template <typename T, bool L>
   void launch_t(const BatchArgs& A, int64_t grid, int kexpl, int j, hipStream_t st) {
-#define PF_BATCH_LAUNCH(VV)                                                          \
-  hipLaunchKernelGGL((qsun_batch_step_kernel<T, VV>), dim3((unsigned)grid), dim3(256), \
-                     0, st, A, kexpl, j)
+    const size_t sh = L ? (size_t)rpb() * rpt * s * sizeof(T) : 0;
+#define PF_BATCH_LAUNCH(VV)                                                                \
+  hipLaunchKernelGGL((qsun_batch_step_kernel<T, VV, L>), dim3((unsigned)grid), dim3(256), sh, \
+                     st, A, kexpl, j)
+    switch (V) {
+      case 2: PF_BATCH_LAUNCH(2); break;
+      case 4: PF_BATCH_LAUNCH(4); break;
+      case 8: PF_BATCH_LAUNCH(8); break;
+      default: PF_BATCH_LAUNCH(1); break;
+    }
+#undef PF_BATCH_LAUNCH
+  }
+  template <bool L>
+  void launch_mixed(const BatchArgs& A, int64_t grid, int kexpl, int j, hipStream_t st) {
+    const size_t sh = L ? (size_t)rpb() * rpt * s * sizeof(double) : 0;
+#define PF_BATCH_LAUNCH(VV)                                                       \
+  hipLaunchKernelGGL((qsun_batch_step_kernel_mixed<VV, L>), dim3((unsigned)grid), \
+                     dim3(256), sh, st, A, kexpl, j)
     switch (V) {
       case 2: PF_BATCH_LAUNCH(2); break;
       case 4: PF_BATCH_LAUNCH(4); break;
@@ -368,21 +542,16 @@ struct pf_batch {
   // one step launch covering the arithmetics present in the application (mask)
   void launch(const BatchArgs& A, int64_t grid, int kexpl, int j, hipStream_t st, int mask) {
     if (grid <= 0) return;
+    const bool L = use_lds();
     if (mask == (kMaskF64 | kMaskF32)) {
-#define PF_BATCH_LAUNCH(VV)                                                             \
-  hipLaunchKernelGGL((qsun_batch_step_kernel_mixed<VV>), dim3((unsigned)grid), dim3(256), \
-                     0, st, A, kexpl, j)
-      switch (V) {
-        case 2: PF_BATCH_LAUNCH(2); break;
-        case 4: PF_BATCH_LAUNCH(4); break;
-        case 8: PF_BATCH_LAUNCH(8); break;
-        default: PF_BATCH_LAUNCH(1); break;
-      }
-#undef PF_BATCH_LAUNCH
+      if (L) launch_mixed<true>(A, grid, kexpl, j, st);
+      else launch_mixed<false>(A, grid, kexpl, j, st);
     } else if (mask & kMaskF64) {
-      launch_t<double>(A, grid, kexpl, j, st);
+      if (L) launch_t<double, true>(A, grid, kexpl, j, st);
+      else launch_t<double, false>(A, grid, kexpl, j, st);
     } else if (mask & kMaskF32) {
-      launch_t<float>(A, grid, kexpl, j, st);
+      if (L) launch_t<float, true>(A, grid, kexpl, j, st);
+      else launch_t<float, false>(A, grid, kexpl, j, st);
     }
     PF_HIP(hipGetLastError());
   }
@@ -600,8 +769,22 @@ extern "C" int pf_batch_set_tuning(pf_batch* b, int32_t key, int32_t value) {
         b->shfl = value;
         b->drop_graph();
         break;
+      case PF_BATCH_LDS_ROWS:
+        PF_REQUIRE(value == 0 || value == 1, "batch: LDS rows 0/1");
+        b->lds = value;
+        b->drop_graph();
+        break;
       default: throw Error{PF_ERR_INVALID, "batch: unknown tuning key"};
     }
+  });
+}
+
+extern "C" int pf_batch_get_tuning(const pf_batch* b, int32_t* cfg) {
+  return guarded([&] {
+    PF_REQUIRE(b && cfg, "batch: null argument");
+    const int32_t v[8] = {b->V,       b->rpt,          b->xcd,         b->shfl,
+                          b->lds,     b->use_lds(),    b->use_graph,   b->gsteps};
+    for (int i = 0; i < 8; ++i) cfg[i] = v[i];
   });
 }
 
@@ -628,16 +811,28 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
     PF_HIP(hipEventCreate(&e0));
     PF_HIP(hipEventCreate(&e1));
     struct Cand {
-      int V, rpt, xcd, shfl;
+      int V, rpt, xcd, shfl, lds;
       double us;
     };
     std::vector<Cand> cands;
     for (int V : {1, 2, 4, 8})
-      if (batch_vec_ok(b->s, V) && !(dtype == PF_F64 && V > 8))
+      if (batch_vec_ok(b->s, V) && !(dtype == PF_F64 && V > 8)) {
         for (int rpt : {1, 4, 16, 64})
           for (int x : {1, 0})
-            for (int sh : {1, 0}) cands.push_back({V, rpt, x, sh, 1e300});
-    const int saveV = b->V, saveR = b->rpt, saveX = b->xcd, saveS = b->shfl;
+            for (int sh : {1, 0}) cands.push_back({V, rpt, x, sh, 0, 1e300});
+        // LDS form (the grain block comes from LDS: no shuffle choice)
+        for (int rpt : {1, 2, 4, 8, 16}) {
+          const int saveV0 = b->V, saveR0 = b->rpt;
+          b->V = V;
+          b->rpt = rpt;
+          const bool ok = b->lds_ok();
+          b->V = saveV0;
+          b->rpt = saveR0;
+          if (ok)
+            for (int x : {1, 0}) cands.push_back({V, rpt, x, 1, 1, 1e300});
+        }
+      }
+    const int saveV = b->V, saveR = b->rpt, saveX = b->xcd, saveS = b->shfl, saveL = b->lds;
     const bool had_filter = b->have_filter;
     for (int round = 0; round < 3; ++round) {
       for (auto& c : cands) {
@@ -645,6 +840,7 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
         b->rpt = c.rpt;
         b->xcd = c.xcd;
         b->shfl = c.shfl;
+        b->lds = c.lds;
         b->layout_blocks();
         BatchArgs A = b->args();
         A.coefs = nullptr;
@@ -661,7 +857,7 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
       }
     }
     double best = 1e300;
-    Cand bc{saveV, saveR, saveX, saveS, 1e300};
+    Cand bc{saveV, saveR, saveX, saveS, saveL, 1e300};
     for (const auto& c : cands)
       if (c.us < 0.99 * best) {
         best = c.us;
@@ -671,6 +867,7 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
     b->rpt = bc.rpt;
     b->xcd = bc.xcd;
     b->shfl = bc.shfl;
+    b->lds = bc.lds;
     b->layout_blocks();
     // graph replays against stream launches for the chosen tiling
     {

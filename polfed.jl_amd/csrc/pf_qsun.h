@@ -163,4 +163,141 @@ __device__ __forceinline__ typename VT<T, V>::t qsun_hrow_dg(const QsunRow& q, u
                     : qsun_hrow_dg2<T, V, false>(q, r, sG, G);
 }
 
+// ----------------------------------------------- block rows staged in LDS
+// A block that owns the aligned row range [r0, r0 + 2^m) of one operator
+// (row stride a power of two) copies that range of b2 to LDS once; the flips
+// whose mask lies below 2^m (outer spins j < m - Lg: the flipped row stays in
+// the range) and the grain block are then read from LDS, and only the other
+// flips go through the vector memory pipeline.
+
+constexpr int kLdsFlipBatch = 4;  // flips per batch in the LDS form (both lists)
+
+// Local flips of an operator with Lg grain and nloc outer spins in a block of
+// 2^m rows.
+__host__ __device__ __forceinline__ int qsun_local_flips(int m, int Lg, int nloc) {
+  const int nl = m - Lg;
+  return nl < 0 ? 0 : (nl > nloc ? nloc : nl);
+}
+
+// Stage sF/sC/sG for the LDS form: the nl local flips first, padded to a
+// multiple of kLdsFlipBatch with zero masks and weights (a read of the row
+// itself that adds nothing), then the nloc - nl others, padded the same way.
+// Masks are shifted by rsh (the row stride is 2^rsh bytes).  Call from all
+// threads, then __syncthreads().
+template <typename T>
+__device__ __forceinline__ void qsun_stage_split(int nloc, int nl, int dg, const uint32_t* flip,
+                                                 const double* coup, const double* G,
+                                                 double scale, int rsh, uint32_t* sF, T* sC,
+                                                 T* sG) {
+  constexpr int B = kLdsFlipBatch;
+  const int nlp = (nl + B - 1) / B * B, nr = nloc - nl, nrp = (nr + B - 1) / B * B;
+  for (int t = threadIdx.x; t < nlp + nrp; t += blockDim.x) {
+    const bool loc = t < nlp;
+    const int i = loc ? t : t - nlp;  // index inside its list
+    const bool real = loc ? i < nl : i < nr;
+    const int j = loc ? i : nl + i;
+    sF[t] = real ? (flip[j] << rsh) : 0u;
+    sC[t] = real ? (T)(coup[j] * scale) : (T)0;
+  }
+  for (int t = threadIdx.x; t < dg * dg; t += blockDim.x) sG[t] = (T)(G[t] * scale);
+}
+
+// Vector load at byte offset off from sB, known to be LDS.
+template <typename T, int V>
+__device__ __forceinline__ typename VT<T, V>::t lds_vload(const char* sB, uint32_t off) {
+  typedef typename VT<T, V>::t vt;
+  typedef const __attribute__((address_space(3))) vt* lp;
+  typedef const __attribute__((address_space(3))) char* lc;
+  return *(lp)((lc)sB + off);
+}
+
+// Rows a lane takes at once in the LDS form: about 16 bytes of each gather
+// per lane and row group (4 FP32 rows, 2 FP64 rows, one row of 4+ columns).
+template <typename T, int V>
+constexpr int qsun_lds_rows() {
+  return (16 / (V * (int)sizeof(T))) < 1 ? 1 : (16 / (V * (int)sizeof(T))) > 4 ? 4
+                                                 : 16 / (V * (int)sizeof(T));
+}
+
+// H~ b2 at the R rows r[i] of this lane (its V columns) in the LDS form, into
+// acc[i].  q.b2/q.cofs_b/q.rsh address the operator's b2 (row stride 2^rsh
+// bytes); sB holds rows [r0, r0 + 2^m) with the same stride and rl_b[i] =
+// ((r[i] - r0) << rsh) + cofs_b; sF/sC are laid out by qsun_stage_split with
+// nlp local and nrp other entries (multiples of kLdsFlipBatch).  The R rows
+// share their grain position a (they differ by multiples of 2^Lg).  Every
+// batch issues its loads for all R rows before the first wait; the other
+// flips go first (their gathers travel while the LDS reads run).
+template <typename T, int V, int DG, int R>
+__device__ __forceinline__ void qsun_hrows_lds(const QsunRow& q, const uint32_t* r,
+                                               const uint32_t* rl_b, const char* sB,
+                                               const T* sG, int nlp, int nrp,
+                                               typename VT<T, V>::t* acc) {
+  typedef typename VT<T, V>::t vt;
+  constexpr int B = kLdsFlipBatch;
+  const T* sC = reinterpret_cast<const T*>(q.sC);
+  const int a = (int)(r[0] & (uint32_t)(DG - 1));
+  double dl[R], dh[R];
+  uint32_t rg_b[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    dl[i] = as_global(q.dlo)[r[i] & ((1u << q.lb) - 1u)];
+    dh[i] = as_global(q.dhi)[r[i] >> q.lb];
+    rg_b[i] = (r[i] << q.rsh) + q.cofs_b;  // the row's byte offset in b2
+    acc[i] = (vt)(T)0;
+  }
+  for (int j0 = nlp; j0 < nlp + nrp; j0 += B) {
+    uint32_t f[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) f[u] = q.sF[j0 + u];
+    vt x[R][B];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int u = 0; u < B; ++u) x[i][u] = gload<T, V>(q.b2, rg_b[i] ^ f[u]);
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int u = 0; u < B; ++u) acc[i] += sC[j0 + u] * x[i][u];
+  }
+  for (int j0 = 0; j0 < nlp; j0 += B) {
+    uint32_t f[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) f[u] = q.sF[j0 + u];
+    vt x[R][B];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int u = 0; u < B; ++u) x[i][u] = lds_vload<T, V>(sB, rl_b[i] ^ f[u]);
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int u = 0; u < B; ++u) acc[i] += sC[j0 + u] * x[i][u];
+  }
+#pragma unroll
+  for (int b = 1; b < DG; ++b) {
+    const T w = sG[a * DG + (a ^ b)];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+      acc[i] += w * lds_vload<T, V>(sB, rl_b[i] ^ ((uint32_t)b << q.rsh));
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const T d = (T)((dl[i] + dh[i] - q.shift) * q.scale);
+    acc[i] += d * lds_vload<T, V>(sB, rl_b[i]);
+  }
+}
+
+template <typename T, int V, int R>
+__device__ __forceinline__ void qsun_hrows_lds_dg(const QsunRow& q, const uint32_t* r,
+                                                  const uint32_t* rl_b, const char* sB,
+                                                  const T* sG, int nlp, int nrp,
+                                                  typename VT<T, V>::t* acc) {
+  switch (q.dg) {  // uniform
+    case 2: qsun_hrows_lds<T, V, 2, R>(q, r, rl_b, sB, sG, nlp, nrp, acc); break;
+    case 4: qsun_hrows_lds<T, V, 4, R>(q, r, rl_b, sB, sG, nlp, nrp, acc); break;
+    case 16: qsun_hrows_lds<T, V, 16, R>(q, r, rl_b, sB, sG, nlp, nrp, acc); break;
+    default: qsun_hrows_lds<T, V, 8, R>(q, r, rl_b, sB, sG, nlp, nrp, acc); break;
+  }
+}
+
 }  // namespace pf

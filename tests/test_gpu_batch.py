@@ -63,6 +63,37 @@ def test_batch_filter_matches_oracle(dev, s, rows_per_lane, shuffle, graph):
             assert _rel(dev.to_host(y), ref) < 1e-12, (rep, i)
 
 
+@pytest.mark.parametrize("s,vec,rows_per_lane,used", [
+    (1, 1, 4, 1), (1, 1, 8, 1), (2, 2, 2, 1), (4, 1, 4, 1), (8, 2, 2, 1), (4, 4, 1, 1),
+    (1, 1, 2, 0), (3, 1, 4, 0)])
+def test_batch_filter_lds_rows_matches_oracle(dev, s, vec, rows_per_lane, used):
+    """Block rows staged in LDS: blocks of 128..2048 rows, so the larger
+    instances gather some flips through memory and read the others from LDS,
+    and the smaller ones lie inside one block.  Rows per lane that are not
+    whole FP32 row groups, or ncols=3 (not a power of two), keep the other
+    form.  FP64, FP32 and both kinds in one application."""
+    from polfed_amd.batch import QSunBatch
+    ops, Hs = _instances(dev)
+    orders = [37, 120, 0, 1, 64]
+    coeffs = [oc.filter_coefficients(0.1 * i - 0.2, K) if K > 0 else np.array([0.7])
+              for i, K in enumerate(orders)]
+    batch = QSunBatch(ops, s)
+    batch.set_filter(coeffs)
+    batch.set_tuning(vec=vec, rows_per_lane=rows_per_lane, lds_rows=1, graph_steps=16)
+    t = batch.tuning()
+    assert t["lds_rows"] == 1 and t["lds_used"] == used
+    rng = np.random.default_rng(5)
+    Xs = [rng.standard_normal((op.D, s)) for op in ops]
+    refs = [_oracle_filter(H, op, c, X) for H, op, c, X in zip(Hs, ops, coeffs, Xs)]
+    tol = {"f64": 1e-12, "f32": 5e-4}
+    for dts in (["f64"] * 5, ["f32"] * 5, ["f32", "f64", "f64", "f32", "f64"]):
+        Xd = [dev.to_device(X) for X in Xs]
+        Yd = [torch.full_like(x, np.nan) for x in Xd]
+        batch.filter([(x, y, dt) for x, y, dt in zip(Xd, Yd, dts)])
+        for i, (y, ref, dt) in enumerate(zip(Yd, refs, dts)):
+            assert _rel(dev.to_host(y), ref) < tol[dt], (dts, i)
+
+
 def test_batch_filter_inactive_and_mixed_dtypes(dev):
     """Idle instances keep their outputs untouched; per-instance FP32/FP64."""
     from polfed_amd.batch import QSunBatch

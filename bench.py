@@ -218,7 +218,10 @@ def batch_filter_only(args, dev):
     [-40, 40] (wider than every instance's spectrum) with a 401-coefficient
     filter, one column each; the autotuned kernel (``--dtype``), then about
     ``--filter-only`` batched steps in stream mode (one kernel per step, so a
-    profiler sees every step)."""
+    profiler sees every step).  ``--batch-form`` other than auto replaces the
+    autotuned form by a fixed one: rows from memory or block rows in LDS with
+    4 (FP64) / 8 (FP32) rows per lane, or row vectors one at a time with one
+    row group per lane."""
     import numpy as np
     import torch
     from polfed_amd.batch import QSunBatch
@@ -231,6 +234,12 @@ def batch_filter_only(args, dev):
     dt = args.dtype
     tuned_us = b.autotune(dt)
     b.set_tuning(graph=0)
+    rpl = 4 if dt == "f64" else 8
+    forms = {"plain": dict(vec=1, rows_per_lane=rpl, lds_rows=0, row_vectors=0),
+             "lds": dict(vec=1, rows_per_lane=rpl, lds_rows=1, row_vectors=0),
+             "rowvec": dict(vec=1, rows_per_lane=1, lds_rows=0, row_vectors=1)}
+    if args.batch_form != "auto":
+        b.set_tuning(**forms[args.batch_form])
     X = [dev.to_device(x) for x in x0s]
     Y = [torch.empty_like(x) for x in X]
     reqs = [(x, y, dt) for x, y in zip(X, Y)]
@@ -1083,6 +1092,9 @@ def main():
     ap.add_argument("--layout", default="", metavar="V,PBLOCKS,WIDE,PARTITION",
                     help="--filter-only: fixed step-kernel launch configuration instead of "
                          "the autotuner's (e.g. 2,8,0,1)")
+    ap.add_argument("--batch-form", default="auto", choices=["auto", "plain", "lds", "rowvec"],
+                    help="--workload qsun-batch --filter-only: the multi-instance kernel form "
+                         "(autotuned, or rows from memory / block rows in LDS / row vectors)")
     ap.add_argument("--row-partition", type=int, default=-1, choices=[-1, 0, 1],
                     help="--filter-only: spin-chain row order (-1 autotuned, 0 reference, "
                          "1 XCD partition)")

@@ -343,7 +343,7 @@ enum {
   PF_BATCH_ROWS_PER_LANE = 4,  /* rows per lane: 1..64                            */
   PF_BATCH_XCD = 5,            /* 1: XCD-contiguous block mapping (default)       */
   PF_BATCH_GRAIN_SHUFFLE = 6,  /* 1: grain block by cross-lane shuffles (default) */
-  PF_BATCH_LDS_ROWS = 7        /* 1: a block stages its rows in LDS and reads the
+  PF_BATCH_LDS_ROWS = 7,       /* 1: a block stages its rows in LDS and reads the
                                   grain block and the flips that stay among them
                                   from there; used when ncols is a power of two,
                                   ncols / vec <= 16, rows per lane a power of two
@@ -351,18 +351,26 @@ enum {
                                   block's rows (256 / (ncols / vec) x rows per
                                   lane) hold <= 32 KiB in FP64; ignored otherwise
                                   (default 0)                                     */
+  PF_BATCH_ROW_VECTORS = 8     /* 1 or 2: with ncols = 1, a lane takes 16 bytes
+                                  of consecutive rows (4 FP32 or 2 FP64) per load,
+                                  1 or 2 such vectors at a time, and a block
+                                  1024 x rows per lane rows, staged in LDS; used
+                                  when vec = 1 and rows per lane is 1, 2 or 4 (2
+                                  or 4 for the value 2), ignored otherwise
+                                  (default 0)                                     */
 };
 /* Launch settings of the batch.  No reference counterpart. */
 int pf_batch_set_tuning(pf_batch* b, int32_t key, int32_t value);
 
-/* The batch's launch settings: cfg[0..7] = columns per lane, rows per lane,
+/* The batch's launch settings: cfg[0..9] = columns per lane, rows per lane,
  * XCD mapping, grain shuffles, LDS rows requested, LDS rows in use (the
- * request and the tiling allow it), graph replays, step nodes per graph.
- * No reference counterpart. */
+ * request and the tiling allow it), graph replays, step nodes per graph, row
+ * vectors requested, row vectors in use.  No reference counterpart. */
 int pf_batch_get_tuning(const pf_batch* b, int32_t* cfg);
 
 /* Times every (columns per lane, rows per lane, XCD mapping, grain-block
- * shuffles or reloads, rows in LDS or not) tiling of the multi-instance step kernel on all
+ * shuffles or reloads, rows in LDS or not, row vectors or not) tiling of the
+ * multi-instance step kernel on all
  * instances in arithmetic `dtype`, then graph replays against stream
  * launches, and keeps the fastest.  best_us
  * (host, may be NULL): time per step of the whole batch.  Overwrites the

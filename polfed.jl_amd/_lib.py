@@ -43,6 +43,7 @@ PF_BATCH_ROWS_PER_LANE = 4
 PF_BATCH_XCD = 5
 PF_BATCH_GRAIN_SHUFFLE = 6
 PF_BATCH_LDS_ROWS = 7
+PF_BATCH_ROW_VECTORS = 8
 
 PF_PROBE_GAUSSIAN = 0
 PF_PROBE_PHASE = 1

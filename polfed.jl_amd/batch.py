@@ -24,7 +24,7 @@ from .operators import QSunOperator, _dtype
 _TUNE = {"graph": _lib.PF_BATCH_GRAPH, "graph_steps": _lib.PF_BATCH_GRAPH_STEPS,
          "vec": _lib.PF_BATCH_VEC, "rows_per_lane": _lib.PF_BATCH_ROWS_PER_LANE,
          "xcd": _lib.PF_BATCH_XCD, "grain_shuffle": _lib.PF_BATCH_GRAIN_SHUFFLE,
-         "lds_rows": _lib.PF_BATCH_LDS_ROWS}
+         "lds_rows": _lib.PF_BATCH_LDS_ROWS, "row_vectors": _lib.PF_BATCH_ROW_VECTORS}
 
 
 class QSunBatch:
@@ -110,19 +110,20 @@ class QSunBatch:
 
     def set_tuning(self, **kw) -> None:
         """graph (0/1), graph_steps, vec, rows_per_lane, xcd, grain_shuffle (0/1),
-        lds_rows (0/1)."""
+        lds_rows (0/1), row_vectors (0/1/2)."""
         for k, v in kw.items():
             if k not in _TUNE:
                 raise ValueError(f"unknown batch tuning key {k!r}")
             check(self.lib.pf_batch_set_tuning(self.b, _TUNE[k], int(v)))
 
     def tuning(self) -> dict:
-        """The launch settings in force (pf_batch_get_tuning); lds_used says
-        whether the LDS form runs (requested and allowed by the tiling)."""
-        cfg = (ctypes.c_int32 * 8)()
+        """The launch settings in force (pf_batch_get_tuning); lds_used and
+        row_vectors_used say whether those forms run (requested and allowed by
+        the tiling; row vectors take precedence)."""
+        cfg = (ctypes.c_int32 * 10)()
         check(self.lib.pf_batch_get_tuning(self.b, cfg))
         keys = ("vec", "rows_per_lane", "xcd", "grain_shuffle", "lds_rows", "lds_used",
-                "graph", "graph_steps")
+                "graph", "graph_steps", "row_vectors", "row_vectors_used")
         return dict(zip(keys, list(cfg)))
 
     def autotune(self, dtype="f32", nsteps: int = 40) -> float:

@@ -89,22 +89,22 @@ __device__ __forceinline__ void poly_ab_dev(int type, int k, double& alpha, doub
 // Buffer slot written at local step g (rotation (3,1,2) of the recurrence).
 __device__ __forceinline__ int rot_slot(int g) { return (3 - g % 3) % 3; }
 
-template <typename T, int V>
-__device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& I,
-                                           const BatchDyn& dy, int k, int64_t lb) {
-  typedef typename VT<T, V>::t vt;
-  __shared__ T sG[kMaxDg * kMaxDg];
-  __shared__ T sC[kMaxLocPad];
-  __shared__ uint32_t sF[kMaxLocPad];
-  const int dg = 1 << I.Lg;
-  const uint32_t rsb = (uint32_t)(A.s * sizeof(T));
-  const int rsh = (rsb & (rsb - 1u)) == 0 ? __ffs(rsb) - 1 : -1;
-  const int npad = qsun_pad(I.Lloc, qsun_flip_batch<T, V>());
-  qsun_stage<T>(I.Lloc, npad, dg, I.flip, I.coup, I.G, I.scale, rsh, sF, sC, sG);
-  __syncthreads();
-
+// One instance's step k: which terms it has, their coefficients and its
+// buffers (the rotation follows the local step g = K - k).
+template <typename T>
+struct StepSetup {
+  bool hasH, hasB3;
+  T cx, ch, cb;
+  const T* X;
+  T* OUT;
+  const char* B2;  // b2 (bytes)
+  const T* B3;
+};
+template <typename T>
+__device__ __forceinline__ StepSetup<T> step_setup(const BatchArgs& A, const BatchInst& I,
+                                                   const BatchDyn& dy, int k) {
+  StepSetup<T> r;
   const int g = dy.K - k;  // local step, >= 0
-  const bool hasH = g >= 1;
   double cxd, chd, cbd;
   const double ck = A.coefs ? A.coefs[I.coef_off + k] : 0.5;
   if (k >= 1) {
@@ -121,15 +121,41 @@ __device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& 
     chd = A.polytype == PF_HERMITE ? 2.0 : 1.0;
     cbd = beta1;
   }
-  const bool hasB3 = g >= 2 && cbd != 0.0;
-  const T cx = (T)cxd, ch = (T)chd, cb = (T)cbd;
-
+  r.hasH = g >= 1;
+  r.hasB3 = g >= 2 && cbd != 0.0;
+  r.cx = (T)cxd;
+  r.ch = (T)chd;
+  r.cb = (T)cbd;
   const int64_t off = I.row0 * (int64_t)A.s * 8;  // byte offset of this instance's region
   char* bufs[3] = {A.B0, A.B1, A.B2};
-  const T* __restrict__ X = reinterpret_cast<const T*>(A.Xb + off);
-  T* __restrict__ OUT = reinterpret_cast<T*>(bufs[rot_slot(g)] + off);
-  const char* B2 = bufs[rot_slot(g + 2)] + off;                                       // g-1
-  const T* __restrict__ B3 = reinterpret_cast<const T*>(bufs[rot_slot(g + 1)] + off);  // g-2
+  r.X = reinterpret_cast<const T*>(A.Xb + off);
+  r.OUT = reinterpret_cast<T*>(bufs[rot_slot(g)] + off);
+  r.B2 = bufs[rot_slot(g + 2)] + off;                         // g-1
+  r.B3 = reinterpret_cast<const T*>(bufs[rot_slot(g + 1)] + off);  // g-2
+  return r;
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void batch_body(const BatchArgs& A, const BatchInst& I,
+                                           const BatchDyn& dy, int k, int64_t lb) {
+  typedef typename VT<T, V>::t vt;
+  __shared__ T sG[kMaxDg * kMaxDg];
+  __shared__ T sC[kMaxLocPad];
+  __shared__ uint32_t sF[kMaxLocPad];
+  const int dg = 1 << I.Lg;
+  const uint32_t rsb = (uint32_t)(A.s * sizeof(T));
+  const int rsh = (rsb & (rsb - 1u)) == 0 ? __ffs(rsb) - 1 : -1;
+  const int npad = qsun_pad(I.Lloc, qsun_flip_batch<T, V>());
+  qsun_stage<T>(I.Lloc, npad, dg, I.flip, I.coup, I.G, I.scale, rsh, sF, sC, sG);
+  __syncthreads();
+
+  const StepSetup<T> st = step_setup<T>(A, I, dy, k);
+  const bool hasH = st.hasH, hasB3 = st.hasB3;
+  const T cx = st.cx, ch = st.ch, cb = st.cb;
+  const T* __restrict__ X = st.X;
+  T* __restrict__ OUT = st.OUT;
+  const char* B2 = st.B2;
+  const T* __restrict__ B3 = st.B3;
 
   const int rloc = threadIdx.x / A.G;
   const int gl = threadIdx.x - rloc * A.G;
@@ -213,33 +239,13 @@ __device__ __forceinline__ void batch_body_lds(const BatchArgs& A, const BatchIn
   const int nlp = (nl + B - 1) / B * B, nrp = (I.Lloc - nl + B - 1) / B * B;
   qsun_stage_split<T>(I.Lloc, nl, dg, I.flip, I.coup, I.G, I.scale, rsh, sF, sC, sG);
 
-  const int g = dy.K - k;  // local step, >= 0
-  const bool hasH = g >= 1;
-  double cxd, chd, cbd;
-  const double ck = A.coefs ? A.coefs[I.coef_off + k] : 0.5;
-  if (k >= 1) {
-    double alpha, beta1, dummy;
-    poly_ab_dev(A.polytype, k, alpha, dummy);
-    poly_ab_dev(A.polytype, k + 1, dummy, beta1);
-    cxd = ck;
-    chd = alpha;
-    cbd = beta1;
-  } else {
-    double dummy, beta1;
-    poly_ab_dev(A.polytype, 1, dummy, beta1);
-    cxd = ck;  // phi0 = 1 for every family
-    chd = A.polytype == PF_HERMITE ? 2.0 : 1.0;
-    cbd = beta1;
-  }
-  const bool hasB3 = g >= 2 && cbd != 0.0;
-  const T cx = (T)cxd, ch = (T)chd, cb = (T)cbd;
-
-  const int64_t off = I.row0 * (int64_t)A.s * 8;  // byte offset of this instance's region
-  char* bufs[3] = {A.B0, A.B1, A.B2};
-  const T* __restrict__ X = reinterpret_cast<const T*>(A.Xb + off);
-  T* __restrict__ OUT = reinterpret_cast<T*>(bufs[rot_slot(g)] + off);
-  const char* B2 = bufs[rot_slot(g + 2)] + off;                                       // g-1
-  const T* __restrict__ B3 = reinterpret_cast<const T*>(bufs[rot_slot(g + 1)] + off);  // g-2
+  const StepSetup<T> st = step_setup<T>(A, I, dy, k);
+  const bool hasH = st.hasH, hasB3 = st.hasB3;
+  const T cx = st.cx, ch = st.ch, cb = st.cb;
+  const T* __restrict__ X = st.X;
+  T* __restrict__ OUT = st.OUT;
+  const char* B2 = st.B2;
+  const T* __restrict__ B3 = st.B3;
 
   const int rloc = threadIdx.x / A.G;
   const int gl = threadIdx.x - rloc * A.G;
@@ -316,16 +322,240 @@ __device__ __forceinline__ void batch_body_lds(const BatchArgs& A, const BatchIn
   }
 }
 
-template <typename T, int V, bool LDS>
+// ------------------------------------------------- row-vector form (ncols 1)
+// Single-column instances: a lane takes W = 16 / sizeof(T) consecutive rows
+// in one 16-byte vector (4 FP32 or 2 FP64 rows).  The flip r ^ f maps the
+// aligned vector v of r to v ^ (f >> log2 W), components permuted by
+// i -> i ^ (f mod W), so one 16-byte load serves W rows: a quarter (FP32) or
+// half (FP64) of the vector-memory instructions and address arithmetic of a
+// lane per row.  A block owns 1024 * rpt rows (rpt 2 or 4), staged in LDS as
+// in the LDS form; each lane takes its vectors two at a time.
+
+template <typename T>
+struct RowVec {
+  static constexpr int W = 16 / (int)sizeof(T);  // rows per vector
+  static constexpr int LW = W == 4 ? 2 : 1;      // log2 W
+  typedef typename VT<T, W>::t t;
+};
+
+// acc += c * x with the components of x permuted by i -> i ^ p (p uniform)
+template <typename T>
+__device__ __forceinline__ void rv_fma_perm(typename RowVec<T>::t& acc, T c,
+                                            const typename RowVec<T>::t& x, uint32_t p) {
+  if constexpr (RowVec<T>::W == 2) {
+    if (p & 1u) {
+      acc[0] += c * x[1];
+      acc[1] += c * x[0];
+    } else {
+      acc += c * x;
+    }
+  } else {
+    switch (p & 3u) {
+      case 0: acc += c * x; break;
+      case 1:
+        acc[0] += c * x[1]; acc[1] += c * x[0]; acc[2] += c * x[3]; acc[3] += c * x[2];
+        break;
+      case 2:
+        acc[0] += c * x[2]; acc[1] += c * x[3]; acc[2] += c * x[0]; acc[3] += c * x[1];
+        break;
+      default:
+        acc[0] += c * x[3]; acc[1] += c * x[2]; acc[2] += c * x[1]; acc[3] += c * x[0];
+        break;
+    }
+  }
+}
+
+// H~ b2 on the R vectors vg[i] (instance vector index; vl[i] the index in
+// the block's LDS copy sB) of this lane, into acc[i].  sF/sC/sG as staged by
+// qsun_stage_split in the row-vector encoding.
+template <typename T, int DG, int R>
+__device__ __forceinline__ void rv_hvec(const char* b2, const char* sB, const uint32_t* vg,
+                                        const uint32_t* vl, const uint32_t* sF, const T* sC,
+                                        const T* sG, int nlp, int nrp, const double* dlo,
+                                        const double* dhi, int lb, double shift, double scale,
+                                        typename RowVec<T>::t* acc) {
+  typedef typename RowVec<T>::t wt;
+  constexpr int W = RowVec<T>::W, LW = RowVec<T>::LW, B = kLdsFlipBatch;
+  double dl[R][W], dh[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const uint32_t row = vg[i] << LW;
+    const uint32_t lo = (row & ((1u << lb) - 1u)) * 8u;  // byte offset in Dlo, 16-byte aligned
+#pragma unroll
+    for (int h = 0; h < W / 2; ++h) {
+      const typename VT<double, 2>::t d2 =
+          gload<double, 2>(reinterpret_cast<const char*>(dlo), lo + 16u * h);
+      dl[i][2 * h] = d2[0];
+      dl[i][2 * h + 1] = d2[1];
+    }
+    dh[i] = as_global(dhi)[row >> lb];
+    acc[i] = (wt)(T)0;
+  }
+  // flips whose rows lie outside the block: gathered from memory
+  for (int j0 = nlp; j0 < nlp + nrp; j0 += B) {
+    uint32_t F[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) F[u] = (uint32_t)__builtin_amdgcn_readfirstlane((int)sF[j0 + u]);
+    wt x[R][B];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int u = 0; u < B; ++u) x[i][u] = gload<T, W>(b2, (vg[i] << 4) ^ (F[u] & ~15u));
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const T c = sC[j0 + u];
+#pragma unroll
+      for (int i = 0; i < R; ++i) rv_fma_perm<T>(acc[i], c, x[i][u], F[u] & 15u);
+    }
+  }
+  // flips inside the block: from LDS
+  for (int j0 = 0; j0 < nlp; j0 += B) {
+    uint32_t F[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) F[u] = (uint32_t)__builtin_amdgcn_readfirstlane((int)sF[j0 + u]);
+    wt x[R][B];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int u = 0; u < B; ++u) x[i][u] = lds_vload<T, W>(sB, (vl[i] << 4) ^ (F[u] & ~15u));
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const T c = sC[j0 + u];
+#pragma unroll
+      for (int i = 0; i < R; ++i) rv_fma_perm<T>(acc[i], c, x[i][u], F[u] & 15u);
+    }
+  }
+  // grain block (zero diagonal in sG) and the diagonal
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const wt own = lds_vload<T, W>(sB, vl[i] << 4);
+    if constexpr (DG >= W) {
+      constexpr int GV = DG / W;  // vectors per grain block
+      const uint32_t vb = vl[i] & ~(uint32_t)(GV - 1);
+      wt xg[GV];
+#pragma unroll
+      for (int t = 0; t < GV; ++t) xg[t] = lds_vload<T, W>(sB, (vb + t) << 4);
+      const int a0 = (int)(vl[i] & (uint32_t)(GV - 1)) << LW;  // grain position of component 0
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        const char* grow = reinterpret_cast<const char*>(sG + (a0 + c) * DG);
+#pragma unroll
+        for (int t = 0; t < GV; ++t) {
+          const wt w = lds_vload<T, W>(grow, 16u * t);
+#pragma unroll
+          for (int e = 0; e < W; ++e) acc[i][c] += w[e] * xg[t][e];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        const int a = c & (DG - 1), base = c & ~(DG - 1);
+#pragma unroll
+        for (int b = 0; b < DG; ++b) acc[i][c] += sG[a * DG + b] * own[base + b];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < W; ++c) acc[i][c] += (T)((dl[i][c] + dh[i] - shift) * scale) * own[c];
+  }
+}
+
+template <typename T, int DG, int R>
+__device__ __forceinline__ void batch_body_rv_dg(const BatchArgs& A, const BatchInst& I,
+                                                 const BatchDyn& dy, int k, int64_t lb, T* sG,
+                                                 T* sC, uint32_t* sF) {
+  typedef typename RowVec<T>::t wt;
+  constexpr int W = RowVec<T>::W, LW = RowVec<T>::LW;
+  const int nl = qsun_local_flips(A.lgrows, I.Lg, I.Lloc);
+  constexpr int B = kLdsFlipBatch;
+  const int nlp = (nl + B - 1) / B * B, nrp = (I.Lloc - nl + B - 1) / B * B;
+  qsun_stage_split<T>(I.Lloc, nl, DG, I.flip, I.coup, I.G, I.scale, 0, sF, sC, sG, LW);
+  const StepSetup<T> st = step_setup<T>(A, I, dy, k);
+
+  const uint32_t tid = threadIdx.x;
+  const int64_t r0 = lb << A.lgrows;                    // first row of the block
+  const uint32_t v0 = (uint32_t)(r0 >> LW);             // its first vector
+  const int nvl = A.rpt * (4 / W);                      // vectors per lane, a multiple of R
+  const uint32_t nvec = (uint32_t)(I.D >> LW);          // vectors of the instance
+  const uint32_t vmask = nvec - 1;                      // (a power of two)
+  typedef __attribute__((address_space(3))) wt lwt;
+  typedef __attribute__((address_space(3))) char lch;
+  if (st.hasH) {  // uniform over the block
+    for (int u0 = 0; u0 < nvl; u0 += R) {
+      wt xr[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+        xr[i] = gload<T, W>(st.B2, ((v0 + (uint32_t)(u0 + i) * 256u + tid) & vmask) << 4);
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+        *reinterpret_cast<lwt*>((lch*)batch_rows + (((uint32_t)(u0 + i) * 256u + tid) << 4)) =
+            xr[i];
+    }
+  }
+  __syncthreads();  // the LDS tables and rows
+
+  const char* Xc = reinterpret_cast<const char*>(st.X);
+  const char* B3c = reinterpret_cast<const char*>(st.B3);
+  for (int u0 = 0; u0 < nvl; u0 += R) {
+    uint32_t vl[R], vg[R];
+    wt xv[R], b3v[R], hv[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      vl[i] = (uint32_t)(u0 + i) * 256u + tid;
+      vg[i] = (v0 + vl[i]) & vmask;  // a block larger than its instance reads row mod D
+      xv[i] = gload<T, W>(Xc, vg[i] << 4);
+      b3v[i] = st.hasB3 ? gload<T, W>(B3c, vg[i] << 4) : (wt)(T)0;
+      hv[i] = (wt)(T)0;
+    }
+    if (st.hasH)
+      rv_hvec<T, DG, R>(st.B2, batch_rows, vg, vl, sF, sC, sG, nlp, nrp, I.qdiag,
+                     I.qdiag + ((int64_t)1 << I.lb), I.lb, I.shift, I.scale, hv);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if (v0 + vl[i] >= nvec) continue;  // past the end of a small instance: no store
+      wt out = st.cx * xv[i];
+      if (st.hasH) out += st.ch * hv[i];
+      if (st.hasB3) out += st.cb * b3v[i];
+      if (k >= 1) {
+        *as_global(reinterpret_cast<wt*>(reinterpret_cast<char*>(st.OUT) + (vg[i] << 4))) = out;
+      } else {
+#pragma unroll
+        for (int c = 0; c < W; ++c)
+          as_global(dy.Y)[((int64_t)vg[i] << LW) + c] = (double)out[c];
+      }
+    }
+  }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void batch_body_rv(const BatchArgs& A, const BatchInst& I,
+                                              const BatchDyn& dy, int k, int64_t lb) {
+  __shared__ __align__(16) T sG[kMaxDg * kMaxDg];
+  __shared__ T sC[kMaxLocPad];
+  __shared__ uint32_t sF[kMaxLocPad];
+  switch (I.Lg) {  // uniform
+    case 1: batch_body_rv_dg<T, 2, R>(A, I, dy, k, lb, sG, sC, sF); break;
+    case 2: batch_body_rv_dg<T, 4, R>(A, I, dy, k, lb, sG, sC, sF); break;
+    case 4: batch_body_rv_dg<T, 16, R>(A, I, dy, k, lb, sG, sC, sF); break;
+    default: batch_body_rv_dg<T, 8, R>(A, I, dy, k, lb, sG, sC, sF); break;
+  }
+}
+
+// Kernel forms: rows per lane from memory (0), block rows in LDS (1), row
+// vectors with block rows in LDS, two (2) or one (3) vectors at a time (one
+// column, V = 1).
+constexpr int kFormPlain = 0, kFormLds = 1, kFormRowVec = 2, kFormRowVec1 = 3;
+
+template <typename T, int V, int FORM>
 __device__ __forceinline__ void batch_body_any(const BatchArgs& A, const BatchInst& I,
                                                const BatchDyn& dy, int k, int64_t lb) {
-  if constexpr (LDS) batch_body_lds<T, V>(A, I, dy, k, lb);
+  if constexpr (FORM == kFormRowVec) batch_body_rv<T, 2>(A, I, dy, k, lb);
+  else if constexpr (FORM == kFormRowVec1) batch_body_rv<T, 1>(A, I, dy, k, lb);
+  else if constexpr (FORM == kFormLds) batch_body_lds<T, V>(A, I, dy, k, lb);
   else batch_body<T, V>(A, I, dy, k, lb);
 }
 
-// One Clenshaw step of every active instance whose arithmetic is T (LDS: the
-// block rows staged in LDS).
-template <typename T, int V, bool LDS>
+// One Clenshaw step of every active instance whose arithmetic is T.
+template <typename T, int V, int FORM>
 __global__ __launch_bounds__(256) void qsun_batch_step_kernel(const BatchArgs A, int kexpl,
                                                               int j) {
   int lo, k;
@@ -333,21 +563,21 @@ __global__ __launch_bounds__(256) void qsun_batch_step_kernel(const BatchArgs A,
   int64_t lb;
   constexpr int kDt = sizeof(T) == 8 ? PF_F64 : PF_F32;
   if (!batch_locate(A, kexpl, j, lo, dy, k, lb) || dy.dtype != kDt) return;
-  batch_body_any<T, V, LDS>(A, A.insts[lo], dy, k, lb);
+  batch_body_any<T, V, FORM>(A, A.insts[lo], dy, k, lb);
 }
 
 // One Clenshaw step of every active instance, FP64 and FP32 instances in the
 // same launch (each block takes its instance's arithmetic): an application
 // with both kinds costs one launch per step, not two over the whole grid.
-template <int V, bool LDS>
+template <int V, int FORM>
 __global__ __launch_bounds__(256) void qsun_batch_step_kernel_mixed(const BatchArgs A, int kexpl,
                                                                     int j) {
   int lo, k;
   BatchDyn dy;
   int64_t lb;
   if (!batch_locate(A, kexpl, j, lo, dy, k, lb)) return;
-  if (dy.dtype == PF_F64) batch_body_any<double, V, LDS>(A, A.insts[lo], dy, k, lb);
-  else batch_body_any<float, V, LDS>(A, A.insts[lo], dy, k, lb);
+  if (dy.dtype == PF_F64) batch_body_any<double, V, FORM>(A, A.insts[lo], dy, k, lb);
+  else batch_body_any<float, V, FORM>(A, A.insts[lo], dy, k, lb);
 }
 
 // dtype bit set of an application: bit 0 FP64 instances, bit 1 FP32 ones
@@ -386,7 +616,7 @@ struct pf_batch {
   bool have_filter = false;
   DevBuf insts, dyn, blk0d, coefs, kstate, bufs;
   // tuning
-  int V = 1, rpt = 1, xcd = 1, use_graph = 1, gsteps = 96, shfl = 1, lds = 0;
+  int V = 1, rpt = 1, xcd = 1, use_graph = 1, gsteps = 96, shfl = 1, lds = 0, rv = 0;
   // pinned staging of the per-application records; the event marks the end
   // of the last copy out of it
   BatchDyn* hdyn = nullptr;
@@ -410,6 +640,18 @@ struct pf_batch {
            (size_t)rows * s * 8 <= kBatchLdsBytes;
   }
   bool use_lds() const { return lds && lds_ok(); }
+  // The row-vector form (rv = vectors a lane takes at once, 1 or 2): one
+  // column, one column per lane, 1024 * rpt rows per block with rpt 1, 2 or
+  // 4 (whole vector pairs per lane in FP32 for rv = 2; FP64 rows within
+  // kBatchLdsBytes).
+  bool rv_ok() const {
+    return s == 1 && V == 1 && (rpt == 1 || rpt == 2 || rpt == 4) && !(rv == 2 && rpt == 1);
+  }
+  bool use_rv() const { return rv && rv_ok(); }
+  int rows_per_block() const { return use_rv() ? 1024 * rpt : rpb() * rpt; }
+  int form() const {
+    return use_rv() ? (rv == 2 ? kFormRowVec : kFormRowVec1) : use_lds() ? kFormLds : kFormPlain;
+  }
   // rows a lane takes at once in the LDS form with FP32 (a multiple of the
   // FP64 figure)
   int lds_rows_f32() const {
@@ -445,7 +687,7 @@ struct pf_batch {
   void layout_blocks() {
     PF_HIP(hipDeviceSynchronize());
     blk0.assign(n + 1, 0);
-    const int64_t per = (int64_t)rpb() * rpt;
+    const int64_t per = rows_per_block();
     for (int p = 0; p < n; ++p) {
       const int64_t D = ops[order[p]]->D;
       blk0[p + 1] = blk0[p] + (D + per - 1) / per;
@@ -508,50 +750,63 @@ struct pf_batch {
     A.rpt = rpt;
     A.xcd = xcd;
     A.shfl = shfl;
-    A.lgrows = __builtin_ctz((unsigned)(rpb() * rpt));
+    A.lgrows = __builtin_ctz((unsigned)rows_per_block());
     return A;
   }
-  template <typename T, bool L>
+  // dynamic LDS of a launch: the block's rows in the LDS forms
+  size_t lds_bytes(size_t elem) const {
+    return form() == kFormPlain ? 0 : (size_t)rows_per_block() * s * elem;
+  }
+  template <typename T, int F>
   void launch_t(const BatchArgs& A, int64_t grid, int kexpl, int j, hipStream_t st) {
-    const size_t sh = L ? (size_t)rpb() * rpt * s * sizeof(T) : 0;
+    const size_t sh = lds_bytes(sizeof(T));
 #define PF_BATCH_LAUNCH(VV)                                                                \
-  hipLaunchKernelGGL((qsun_batch_step_kernel<T, VV, L>), dim3((unsigned)grid), dim3(256), sh, \
+  hipLaunchKernelGGL((qsun_batch_step_kernel<T, VV, F>), dim3((unsigned)grid), dim3(256), sh, \
                      st, A, kexpl, j)
-    switch (V) {
-      case 2: PF_BATCH_LAUNCH(2); break;
-      case 4: PF_BATCH_LAUNCH(4); break;
-      case 8: PF_BATCH_LAUNCH(8); break;
-      default: PF_BATCH_LAUNCH(1); break;
+    if constexpr (F == kFormRowVec || F == kFormRowVec1) {
+      PF_BATCH_LAUNCH(1);
+    } else {
+      switch (V) {
+        case 2: PF_BATCH_LAUNCH(2); break;
+        case 4: PF_BATCH_LAUNCH(4); break;
+        case 8: PF_BATCH_LAUNCH(8); break;
+        default: PF_BATCH_LAUNCH(1); break;
+      }
     }
 #undef PF_BATCH_LAUNCH
   }
-  template <bool L>
+  template <int F>
   void launch_mixed(const BatchArgs& A, int64_t grid, int kexpl, int j, hipStream_t st) {
-    const size_t sh = L ? (size_t)rpb() * rpt * s * sizeof(double) : 0;
+    const size_t sh = lds_bytes(sizeof(double));
 #define PF_BATCH_LAUNCH(VV)                                                       \
-  hipLaunchKernelGGL((qsun_batch_step_kernel_mixed<VV, L>), dim3((unsigned)grid), \
+  hipLaunchKernelGGL((qsun_batch_step_kernel_mixed<VV, F>), dim3((unsigned)grid), \
                      dim3(256), sh, st, A, kexpl, j)
-    switch (V) {
-      case 2: PF_BATCH_LAUNCH(2); break;
-      case 4: PF_BATCH_LAUNCH(4); break;
-      case 8: PF_BATCH_LAUNCH(8); break;
-      default: PF_BATCH_LAUNCH(1); break;
+    if constexpr (F == kFormRowVec || F == kFormRowVec1) {
+      PF_BATCH_LAUNCH(1);
+    } else {
+      switch (V) {
+        case 2: PF_BATCH_LAUNCH(2); break;
+        case 4: PF_BATCH_LAUNCH(4); break;
+        case 8: PF_BATCH_LAUNCH(8); break;
+        default: PF_BATCH_LAUNCH(1); break;
+      }
     }
 #undef PF_BATCH_LAUNCH
+  }
+  template <int F>
+  void launch_f(const BatchArgs& A, int64_t grid, int kexpl, int j, hipStream_t st, int mask) {
+    if (mask == (kMaskF64 | kMaskF32)) launch_mixed<F>(A, grid, kexpl, j, st);
+    else if (mask & kMaskF64) launch_t<double, F>(A, grid, kexpl, j, st);
+    else if (mask & kMaskF32) launch_t<float, F>(A, grid, kexpl, j, st);
   }
   // one step launch covering the arithmetics present in the application (mask)
   void launch(const BatchArgs& A, int64_t grid, int kexpl, int j, hipStream_t st, int mask) {
     if (grid <= 0) return;
-    const bool L = use_lds();
-    if (mask == (kMaskF64 | kMaskF32)) {
-      if (L) launch_mixed<true>(A, grid, kexpl, j, st);
-      else launch_mixed<false>(A, grid, kexpl, j, st);
-    } else if (mask & kMaskF64) {
-      if (L) launch_t<double, true>(A, grid, kexpl, j, st);
-      else launch_t<double, false>(A, grid, kexpl, j, st);
-    } else if (mask & kMaskF32) {
-      if (L) launch_t<float, true>(A, grid, kexpl, j, st);
-      else launch_t<float, false>(A, grid, kexpl, j, st);
+    switch (form()) {
+      case kFormRowVec: launch_f<kFormRowVec>(A, grid, kexpl, j, st, mask); break;
+      case kFormRowVec1: launch_f<kFormRowVec1>(A, grid, kexpl, j, st, mask); break;
+      case kFormLds: launch_f<kFormLds>(A, grid, kexpl, j, st, mask); break;
+      default: launch_f<kFormPlain>(A, grid, kexpl, j, st, mask); break;
     }
     PF_HIP(hipGetLastError());
   }
@@ -774,6 +1029,11 @@ extern "C" int pf_batch_set_tuning(pf_batch* b, int32_t key, int32_t value) {
         b->lds = value;
         b->drop_graph();
         break;
+      case PF_BATCH_ROW_VECTORS:
+        PF_REQUIRE(value >= 0 && value <= 2, "batch: row vectors 0/1/2");
+        b->rv = value;
+        b->layout_blocks();
+        break;
       default: throw Error{PF_ERR_INVALID, "batch: unknown tuning key"};
     }
   });
@@ -782,9 +1042,9 @@ extern "C" int pf_batch_set_tuning(pf_batch* b, int32_t key, int32_t value) {
 extern "C" int pf_batch_get_tuning(const pf_batch* b, int32_t* cfg) {
   return guarded([&] {
     PF_REQUIRE(b && cfg, "batch: null argument");
-    const int32_t v[8] = {b->V,       b->rpt,          b->xcd,         b->shfl,
-                          b->lds,     b->use_lds(),    b->use_graph,   b->gsteps};
-    for (int i = 0; i < 8; ++i) cfg[i] = v[i];
+    const int32_t v[10] = {b->V,   b->rpt,       b->xcd,       b->shfl,   b->lds,
+                           b->use_lds(), b->use_graph, b->gsteps, b->rv, b->use_rv()};
+    for (int i = 0; i < 10; ++i) cfg[i] = v[i];
   });
 }
 
@@ -811,7 +1071,7 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
     PF_HIP(hipEventCreate(&e0));
     PF_HIP(hipEventCreate(&e1));
     struct Cand {
-      int V, rpt, xcd, shfl, lds;
+      int V, rpt, xcd, shfl, lds, rv;
       double us;
     };
     std::vector<Cand> cands;
@@ -819,7 +1079,7 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
       if (batch_vec_ok(b->s, V) && !(dtype == PF_F64 && V > 8)) {
         for (int rpt : {1, 4, 16, 64})
           for (int x : {1, 0})
-            for (int sh : {1, 0}) cands.push_back({V, rpt, x, sh, 0, 1e300});
+            for (int sh : {1, 0}) cands.push_back({V, rpt, x, sh, 0, 0, 1e300});
         // LDS form (the grain block comes from LDS: no shuffle choice)
         for (int rpt : {1, 2, 4, 8, 16}) {
           const int saveV0 = b->V, saveR0 = b->rpt;
@@ -829,10 +1089,16 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
           b->V = saveV0;
           b->rpt = saveR0;
           if (ok)
-            for (int x : {1, 0}) cands.push_back({V, rpt, x, 1, 1, 1e300});
+            for (int x : {1, 0}) cands.push_back({V, rpt, x, 1, 1, 0, 1e300});
         }
       }
-    const int saveV = b->V, saveR = b->rpt, saveX = b->xcd, saveS = b->shfl, saveL = b->lds;
+    if (b->s == 1)  // row-vector forms
+      for (int rv : {1, 2})
+        for (int rpt : {1, 2, 4})
+          if (!(rv == 2 && rpt == 1))
+            for (int x : {1, 0}) cands.push_back({1, rpt, x, 1, 0, rv, 1e300});
+    const int saveV = b->V, saveR = b->rpt, saveX = b->xcd, saveS = b->shfl, saveL = b->lds,
+              saveRV = b->rv;
     const bool had_filter = b->have_filter;
     for (int round = 0; round < 3; ++round) {
       for (auto& c : cands) {
@@ -841,6 +1107,7 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
         b->xcd = c.xcd;
         b->shfl = c.shfl;
         b->lds = c.lds;
+        b->rv = c.rv;
         b->layout_blocks();
         BatchArgs A = b->args();
         A.coefs = nullptr;
@@ -857,7 +1124,7 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
       }
     }
     double best = 1e300;
-    Cand bc{saveV, saveR, saveX, saveS, saveL, 1e300};
+    Cand bc{saveV, saveR, saveX, saveS, saveL, saveRV, 1e300};
     for (const auto& c : cands)
       if (c.us < 0.99 * best) {
         best = c.us;
@@ -868,6 +1135,7 @@ extern "C" int pf_batch_autotune(pf_batch* b, int32_t dtype, int32_t nsteps, dou
     b->xcd = bc.xcd;
     b->shfl = bc.shfl;
     b->lds = bc.lds;
+    b->rv = bc.rv;
     b->layout_blocks();
     // graph replays against stream launches for the chosen tiling
     {

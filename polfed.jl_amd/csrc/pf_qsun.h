@@ -184,11 +184,18 @@ __host__ __device__ __forceinline__ int qsun_local_flips(int m, int Lg, int nloc
 // itself that adds nothing), then the nloc - nl others, padded the same way.
 // Masks are shifted by rsh (the row stride is 2^rsh bytes).  Call from all
 // threads, then __syncthreads().
+//
+// Row-vector encoding (lw >= 1, the row stride one element): a lane holds 2^lw
+// consecutive rows in one 16-byte vector, and the flip r ^ f maps the vector v
+// of r to v ^ (f >> lw) with its components permuted by i -> i ^ (f mod 2^lw);
+// sF then holds ((f >> lw) << 4) | (f mod 2^lw), the byte XOR of the vector
+// with the permutation in its four low bits, and the grain matrix in sG has a
+// zero diagonal (the diagonal is d(r)).
 template <typename T>
 __device__ __forceinline__ void qsun_stage_split(int nloc, int nl, int dg, const uint32_t* flip,
                                                  const double* coup, const double* G,
                                                  double scale, int rsh, uint32_t* sF, T* sC,
-                                                 T* sG) {
+                                                 T* sG, int lw = 0) {
   constexpr int B = kLdsFlipBatch;
   const int nlp = (nl + B - 1) / B * B, nr = nloc - nl, nrp = (nr + B - 1) / B * B;
   for (int t = threadIdx.x; t < nlp + nrp; t += blockDim.x) {
@@ -196,10 +203,12 @@ __device__ __forceinline__ void qsun_stage_split(int nloc, int nl, int dg, const
     const int i = loc ? t : t - nlp;  // index inside its list
     const bool real = loc ? i < nl : i < nr;
     const int j = loc ? i : nl + i;
-    sF[t] = real ? (flip[j] << rsh) : 0u;
+    const uint32_t f = real ? flip[j] : 0u;
+    sF[t] = lw ? (((f >> lw) << 4) | (f & ((1u << lw) - 1u))) : (f << rsh);
     sC[t] = real ? (T)(coup[j] * scale) : (T)0;
   }
-  for (int t = threadIdx.x; t < dg * dg; t += blockDim.x) sG[t] = (T)(G[t] * scale);
+  for (int t = threadIdx.x; t < dg * dg; t += blockDim.x)
+    sG[t] = (lw && t / dg == t % dg) ? (T)0 : (T)(G[t] * scale);
 }
 
 // Vector load at byte offset off from sB, known to be LDS.

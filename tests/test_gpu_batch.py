@@ -94,6 +94,43 @@ def test_batch_filter_lds_rows_matches_oracle(dev, s, vec, rows_per_lane, used):
             assert _rel(dev.to_host(y), ref) < tol[dt], (dts, i)
 
 
+@pytest.mark.parametrize("row_vectors,rows_per_lane,used", [
+    (1, 1, 1), (1, 2, 1), (1, 4, 1), (2, 2, 1), (2, 4, 1), (2, 1, 0)])
+def test_batch_filter_row_vectors_matches_oracle(dev, row_vectors, rows_per_lane, used):
+    """Row-vector form (one column; a lane loads 16 bytes of consecutive rows
+    and permutes them per flip): blocks of 1024..4096 rows, so the small
+    instances lie inside one block (rows past the end read row mod D and
+    store nothing) and the larger ones gather their far flips from memory;
+    grain sizes 2, 4 and 8 take both grain paths.  One row per lane with two
+    vectors at a time keeps the other form.  FP64, FP32 and both
+    kinds in one application, graph and stream mode."""
+    from polfed_amd.batch import QSunBatch
+    specs = INSTANCES + [(9, 3, 16), (10, 2, 17), (9, 1, 18)]   # D up to 4096, grain 2
+    ops, Hs = _instances(dev, specs)
+    orders = [37, 120, 0, 1, 64, 90, 45, 70]
+    coeffs = [oc.filter_coefficients(0.1 * i - 0.3, K) if K > 0 else np.array([0.7])
+              for i, K in enumerate(orders)]
+    batch = QSunBatch(ops, 1)
+    batch.set_filter(coeffs)
+    batch.set_tuning(vec=1, rows_per_lane=rows_per_lane, row_vectors=row_vectors,
+                     graph_steps=16)
+    t = batch.tuning()
+    assert t["row_vectors_used"] == used
+    rng = np.random.default_rng(9)
+    Xs = [rng.standard_normal((op.D, 1)) for op in ops]
+    refs = [_oracle_filter(H, op, c, X) for H, op, c, X in zip(Hs, ops, coeffs, Xs)]
+    tol = {"f64": 1e-12, "f32": 5e-4}
+    n = len(ops)
+    for graph in (1, 0):
+        batch.set_tuning(graph=graph)
+        for dts in (["f64"] * n, ["f32"] * n, [("f32", "f64")[i % 2] for i in range(n)]):
+            Xd = [dev.to_device(X) for X in Xs]
+            Yd = [torch.full_like(x, np.nan) for x in Xd]
+            batch.filter([(x, y, dt) for x, y, dt in zip(Xd, Yd, dts)])
+            for i, (y, ref, dt) in enumerate(zip(Yd, refs, dts)):
+                assert _rel(dev.to_host(y), ref) < tol[dt], (graph, dts[i], i)
+
+
 def test_batch_filter_inactive_and_mixed_dtypes(dev):
     """Idle instances keep their outputs untouched; per-instance FP32/FP64."""
     from polfed_amd.batch import QSunBatch

@@ -863,6 +863,7 @@ def j1j2_l24(args):
     d = np.abs(mus["f32"] - mus["f64"]) / op.D
     out["kpm_moment_err_max_over_D"] = float(d.max())
     out["kpm_moment_err_median_over_D"] = float(np.median(d))
+    out["kpm_per_moment_err_over_D"] = [float(f"{e:.1g}") for e in d]
     a, b = (Emax - Emin) / 2, (Emax + Emin) / 2
     rho = spc.DensityOfStates(mus["f64"], "Jackson")
     plan = spc.build_transform_plan(TransformConfig(), a, b, 300, "maxdos")

@@ -400,7 +400,8 @@ __device__ __forceinline__ void rv_hvec(const char* b2, const char* sB, const ui
 #pragma unroll
     for (int i = 0; i < R; ++i)
 #pragma unroll
-      for (int u = 0; u < B; ++u) x[i][u] = gload<T, W>(b2, (vg[i] << 4) ^ (F[u] & ~15u));
+      for (int u = 0; u < B; ++u)  // a padding entry (F = 0) issues no gather
+        x[i][u] = F[u] ? gload<T, W>(b2, (vg[i] << 4) ^ (F[u] & ~15u)) : (wt)(T)0;
 #pragma unroll
     for (int u = 0; u < B; ++u) {
       const T c = sC[j0 + u];

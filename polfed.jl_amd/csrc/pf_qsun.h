@@ -255,14 +255,15 @@ __device__ __forceinline__ void qsun_hrows_lds(const QsunRow& q, const uint32_t*
     acc[i] = (vt)(T)0;
   }
   for (int j0 = nlp; j0 < nlp + nrp; j0 += B) {
+    // masks are uniform: a padding entry (mask 0) issues no gather
     uint32_t f[B];
 #pragma unroll
-    for (int u = 0; u < B; ++u) f[u] = q.sF[j0 + u];
+    for (int u = 0; u < B; ++u) f[u] = (uint32_t)__builtin_amdgcn_readfirstlane((int)q.sF[j0 + u]);
     vt x[R][B];
 #pragma unroll
     for (int i = 0; i < R; ++i)
 #pragma unroll
-      for (int u = 0; u < B; ++u) x[i][u] = gload<T, V>(q.b2, rg_b[i] ^ f[u]);
+      for (int u = 0; u < B; ++u) x[i][u] = f[u] ? gload<T, V>(q.b2, rg_b[i] ^ f[u]) : (vt)(T)0;
 #pragma unroll
     for (int i = 0; i < R; ++i)
 #pragma unroll

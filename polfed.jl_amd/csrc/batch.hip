@@ -18,6 +18,12 @@
 // Each instance has its own arithmetic (FP64 or FP32) per application, so a
 // mixed-precision driver can switch instances independently.  Buffers are
 // sized for FP64; an FP32 instance uses the first half of its region.
+//
+// Three kernel forms, chosen by the autotuner: rows from memory (each lane
+// gathers its row's grain block and outer flips), block rows staged in LDS
+// (the grain block and the flips that stay inside the block's aligned row
+// range read from LDS), and row vectors for one column (a lane takes 16
+// bytes of consecutive rows per load; blocks staged in LDS as well).
 
 #include <algorithm>
 #include <cmath>
@@ -223,7 +229,7 @@ __device__ __forceinline__ bool batch_locate(const BatchArgs& A, int kexpl, int 
 // batch_body with the block's rows of b2 staged in LDS: the block owns the
 // aligned range [r0, r0 + 2^lgrows) of its instance (G a power of two, so
 // every thread has a row), copies it to LDS, and reads the grain block and
-// the flips that stay in the range from there (qsun_hrow_lds).
+// the flips that stay in the range from there (qsun_hrows_lds).
 template <typename T, int V>
 __device__ __forceinline__ void batch_body_lds(const BatchArgs& A, const BatchInst& I,
                                                const BatchDyn& dy, int k, int64_t lb) {

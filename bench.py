@@ -876,6 +876,10 @@ def j1j2_l24(args):
                     generator=torch.Generator(device="cuda").manual_seed(9))
     y = torch.empty_like(q)
     iters = -(-out["maxdim"] // 32)
+    if args.j1j2_iters > 0:
+        out["partial_solve"] = j1j2_partial_solve(dev, op, coeffs, args.j1j2_iters, iters)
+        print(json.dumps(out), flush=True)
+        return
     for dt in ("f32", "f64"):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -889,6 +893,43 @@ def j1j2_l24(args):
                                "projected_solve_filter_s": t * iters}
         log(f"filter {dt}: {t:.1f} s ({t / steps * 1e6:.1f} us/step)")
     print(json.dumps(out), flush=True)
+
+
+def j1j2_partial_solve(dev, op, coeffs, n_iter, full_iters):
+    """The first ``n_iter`` iterations of the nev=300 block-32 Lanczos run on
+    the FP32 filter (FP64 three-term step, CGS2 reorthogonalization and
+    convergence checks as in polfed), timed per stage; the whole run of
+    ``full_iters`` iterations is projected by scaling the measured time per
+    iteration (the reorthogonalization, which grows with the basis, is under
+    1 % of it)."""
+    import numpy as np
+    import torch
+    from polfed_amd.lanczos import lanczos
+    x0 = torch.randn((32, op.D), dtype=torch.float64, device="cuda",
+                     generator=torch.Generator(device="cuda").manual_seed(11))
+    x0 = torch.linalg.qr(x0.T)[0].T.contiguous()
+    done = {"n": 0}
+
+    def f32_filter(X, Y):
+        op.filter(coeffs, X, Y, dtype="f32")
+        done["n"] += 1
+        log(f"l24 partial solve: filter {done['n']}/{n_iter}")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = lanczos(dev, f32_filter, x0, 300, maxdim=32 * n_iter, which="LR", tol=1e-14,
+                  eigentol=1e-10, mapvals=op)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    per_filter = res.times.get("mapping", 0.0) / max(res.numiter, 1)
+    return {"iterations": int(res.numiter), "krylov_dim": int(res.krylovdim), "time_s": t,
+            "stage_s": {k: float(v) for k, v in res.times.items()},
+            "filter_s_per_iteration": per_filter,
+            "lanczos_converged_at_stop": int(res.converged),
+            "projected_full_solve_s": t / max(res.numiter, 1) * full_iters,
+            "note": (f"{res.numiter} of {full_iters} iterations run; the projection scales "
+                     "the measured time per iteration (filter, reorthogonalization at the "
+                     "sizes reached, checks) to the whole run")}
 
 
 def copy_calibration():
@@ -1093,6 +1134,10 @@ def main():
     ap.add_argument("--layout", default="", metavar="V,PBLOCKS,WIDE,PARTITION",
                     help="--filter-only: fixed step-kernel launch configuration instead of "
                          "the autotuner's (e.g. 2,8,0,1)")
+    ap.add_argument("--j1j2-iters", type=int, default=0, metavar="N",
+                    help="--workload j1j2-l24: run the first N iterations of the nev=300 "
+                         "solve on the FP32 filter (timed per stage) instead of the FP32 and "
+                         "FP64 single filter applications")
     ap.add_argument("--batch-form", default="auto", choices=["auto", "plain", "lds", "rowvec"],
                     help="--workload qsun-batch --filter-only: the multi-instance kernel form "
                          "(autotuned, or rows from memory / block rows in LDS / row vectors)")

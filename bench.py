@@ -489,7 +489,7 @@ def _pair_errors(vals, res, ref_vals, width):
     return np.array(err), np.asarray(res)
 
 
-def cpu_baseline_c3(op, rep, x0_host, budget_s=8.0):
+def cpu_baseline_l22(op, rep, x0_host, budget_s=8.0):
     """The oracle's C restatement of the reference's packed CPU filter on a
     bounded prefix of one filter application of the solve's polynomial on the
     16-column block, projected to the FP64 solve's (K+1) x iterations: once
@@ -548,7 +548,7 @@ def cpu_baseline_c3(op, rep, x0_host, budget_s=8.0):
     return out
 
 
-def c1_full_solves(dev):
+def xxz_l14_full_solves(dev):
     """The reference's CPU-runnable case, run in full on both sides: XXZ L=14
     Sz=0 (D=3432), W=5 seed 0, single vector, nev=50, target middle, reference
     defaults, eigentol 1e-10: the oracle's complete CPU solve (one thread for
@@ -727,8 +727,8 @@ def precision_bench(args):
                          "traffic": traffic, "kernel": kname, "layout": klayout,
                          "per_launch_bytes": per_launch, "avg_launch_ms": avg_launch_ms}}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline_c3(op, rep64, x0_host)
-        line["cpu_baseline"]["c1_full_solve"] = c1_full_solves(dev)
+        line["cpu_baseline"] = cpu_baseline_l22(op, rep64, x0_host)
+        line["cpu_baseline"]["xxz_l14_full_solve"] = xxz_l14_full_solves(dev)
     print(json.dumps(line, separators=(",", ":")), flush=True)
 
 

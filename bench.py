@@ -308,16 +308,22 @@ def packed_compare(args):
         print(json.dumps({"mode": "packed-compare", "L": L, "fp64_filter_rel_diff": diff}), flush=True)
 
 
+# (D, k, s) of the reorthogonalization profiles: the last one of each solve
+# (XXZ L=20 block 8, XXZ L=22 block 16, J1-J2 L=24 block 32)
+REORTH_SHAPES = {"l20": (184756, 416, 8), "l22": (705432, 832, 16), "l24": (2704156, 1440, 32)}
+
+
 def reorth_only(args):
     """Profiling driver: CGS2 reorthogonalization W -= V (V^T W), twice, of an
-    8-column block against k=416 basis vectors at D=184756 (the basis size the
-    bench solve reaches), timed with HIP events; prints FP64 MFMA TFLOP/s and
-    algorithmic GB/s (per pass: 4 D k s flops, 8 (2 D k + 3 D s) bytes)."""
+    s-column block against k basis vectors of length D (``--reorth-shape``:
+    the last reorthogonalization of the XXZ L=20, XXZ L=22 or J1-J2 L=24
+    solve), timed with HIP events; prints FP64 MFMA TFLOP/s and algorithmic
+    GB/s (per pass: 4 D k s flops, 8 (2 D k + 3 D s) bytes)."""
     import torch
     from polfed_amd import blockops as bo
     from polfed_amd.device import Device
     dev = Device(0)
-    D, k, s = 184756, 416, 8
+    D, k, s = REORTH_SHAPES[args.reorth_shape]
     g = torch.Generator(device="cuda").manual_seed(7)
     V = torch.randn((k, D), dtype=torch.float64, device="cuda", generator=g)
     W = torch.randn((s, D), dtype=torch.float64, device="cuda", generator=g)
@@ -1146,6 +1152,9 @@ def main():
                          "1 XCD partition)")
     ap.add_argument("--reorth-only", type=int, default=0, metavar="CALLS",
                     help="profiling driver: time CALLS CGS2 reorthogonalizations")
+    ap.add_argument("--reorth-shape", default="l20", choices=sorted(REORTH_SHAPES),
+                    help="--reorth-only: (D, k, s) of the XXZ L=20, XXZ L=22 or J1-J2 L=24 "
+                         "solve's last reorthogonalization")
     ap.add_argument("--workload", default="xxz-l22",
                     choices=["xxz-l22", "xxz-l20", "xxz-l14", "qsun-batch", "j1j2-l24"],
                     help="xxz-l22: the bench line (FP64 and FP32-filter solves, FP32 filter "

@@ -32,6 +32,10 @@ constexpr int kMaxN = 64;            // widest B block of the Gram/update kernel
 // Gram partials: part[(rb * m + i) * n + j] = sum over rb's slabs of A[:, i] B[:, j].
 // Grid: (kGramRowBlocks, ceil(m/64)); block: 4 waves, wave w owns A columns
 // i0 = 64*blockIdx.y + 16*w .. +15 and all n columns of B (NT <= 4 tiles).
+// Slab sl (16 rows) belongs to row block sl mod gridDim.x; a block takes its
+// full slabs two at a time with every load issued before the first MFMA.
+// Lanes past m or n read a clamped column (their results are not stored);
+// only the rows past D of the last, partial slab are zeroed.
 template <int NT>
 __global__ __launch_bounds__(256) void gram_kernel(const double* __restrict__ A, int64_t lda, int m,
                                                    const double* __restrict__ B, int64_t ldb, int n,
@@ -40,47 +44,64 @@ __global__ __launch_bounds__(256) void gram_kernel(const double* __restrict__ A,
   const int i0 = blockIdx.y * 64 + w * 16;
   const int li = lane & 15, kq = lane >> 4;
   const int ia = i0 + li;
+  const double* pa = A + (int64_t)(ia < m ? ia : m - 1) * lda;
+  const double* pb[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int jb = t * 16 + li;
+    pb[t] = B + (int64_t)(jb < n ? jb : n - 1) * ldb;
+  }
   d4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = (d4){0.0, 0.0, 0.0, 0.0};
-  const int64_t nslabs = (D + 15) / 16;
-  const bool full = (D % 16) == 0;
-  for (int64_t sl = blockIdx.x; sl < nslabs; sl += gridDim.x) {
-    const int64_t r0 = sl * 16 + 4 * kq;  // this lane's 4 rows
-    double av[4];
-    if (ia < m) {
-      const double* pa = A + (int64_t)ia * lda + r0;
-      if (full || r0 + 3 < D) {
+  const int64_t gx = gridDim.x, nfull = D / 16;
+  int64_t sl = blockIdx.x;
+  for (; sl + gx < nfull; sl += 2 * gx) {
+    double av[2][4], bv[2][NT][4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) av[q] = pa[q];
-      } else {
+    for (int h = 0; h < 2; ++h) {
+      const int64_t r0 = (sl + h * gx) * 16 + 4 * kq;  // this lane's 4 rows
 #pragma unroll
-        for (int q = 0; q < 4; ++q) av[q] = (r0 + q < D) ? pa[q] : 0.0;
-      }
-    } else {
+      for (int q = 0; q < 4; ++q) av[h][q] = pa[r0 + q];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) av[q] = 0.0;
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bv[h][t][q] = pb[t][r0 + q];
     }
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int jb = t * 16 + li;
-      double bv[4];
-      if (jb < n) {
-        const double* pb = B + (int64_t)jb * ldb + r0;
-        if (full || r0 + 3 < D) {
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) bv[q] = pb[q];
-        } else {
+      for (int t = 0; t < NT; ++t)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) bv[q] = (r0 + q < D) ? pb[q] : 0.0;
-        }
-      } else {
+        for (int q = 0; q < 4; ++q) acc[t] = mfma16x16x4(av[h][q], bv[h][t][q], acc[t]);
+  }
+  if (sl < nfull) {  // one full slab left
+    const int64_t r0 = sl * 16 + 4 * kq;
+    double av[4], bv[NT][4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) bv[q] = 0.0;
-      }
+    for (int q = 0; q < 4; ++q) av[q] = pa[r0 + q];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[t] = mfma16x16x4(av[q], bv[q], acc[t]);
-    }
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bv[t][q] = pb[t][r0 + q];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[t] = mfma16x16x4(av[q], bv[t][q], acc[t]);
+  }
+  if (D % 16 != 0 && blockIdx.x == nfull % gx) {  // the partial last slab
+    const int64_t r0 = nfull * 16 + 4 * kq;
+    double av[4], bv[NT][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) av[q] = (r0 + q < D) ? pa[r0 + q] : 0.0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bv[t][q] = (r0 + q < D) ? pb[t][r0 + q] : 0.0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[t] = mfma16x16x4(av[q], bv[t][q], acc[t]);
   }
   // D-fragment: row (A column) i0 + kq + 4*v, column (B column) 16*t + li
   double* out = part + (int64_t)blockIdx.x * m * n;
@@ -95,21 +116,37 @@ __global__ __launch_bounds__(256) void gram_kernel(const double* __restrict__ A,
   }
 }
 
-// G[i + j*ldg] = sum_rb part[(rb*m + i)*n + j]  (fixed order: deterministic)
+// G[i + j*ldg] = sum_rb part[(rb*m + i)*n + j]  (fixed order: deterministic).
+// The partials are loaded 16 at a time before they are added in order, so
+// one memory round trip serves 16 terms.
 __global__ void gram_reduce_kernel(const double* __restrict__ part, int nrb, int m, int n,
                                    double* __restrict__ G, int ldg) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)m * n) return;
   const int i = (int)(idx / n), j = (int)(idx % n);
+  const int64_t stride = (int64_t)m * n;
+  const double* p = part + (int64_t)i * n + j;
+  constexpr int U = 16;
   double t = 0.0;
-  for (int rb = 0; rb < nrb; ++rb) t += part[((int64_t)rb * m + i) * n + j];
+  int rb = 0;
+  for (; rb + U <= nrb; rb += U, p += U * stride) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = p[u * stride];
+#pragma unroll
+    for (int u = 0; u < U; ++u) t += v[u];
+  }
+  for (; rb < nrb; ++rb, p += stride) t += *p;
   G[i + (int64_t)j * ldg] = t;
 }
 
 // B = beta*B + alpha*A*G.  Block: 4 waves, 64 rows (wave w: rows r0+16w..+15),
 // all n output columns (NT tiles of 16).  Contraction over the m columns of A
 // in steps of 4: A fragment lane l -> A[r0 + l%16, k + l/16] (16 consecutive
-// rows of one column), G fragment lane l -> G[k + l/16, 16t + l%16].
+// rows of one column), G fragment lane l -> G[k + l/16, 16t + l%16].  Four
+// steps' loads are issued before their MFMAs.  Lanes past D or n read a
+// clamped row or column (their results are not stored); only the contraction
+// slots past m in the last steps are zeroed.
 template <int NT>
 __global__ __launch_bounds__(256) void update_kernel(const double* __restrict__ A, int64_t lda,
                                                      int m, const double* __restrict__ G, int ldg,
@@ -119,19 +156,39 @@ __global__ __launch_bounds__(256) void update_kernel(const double* __restrict__ 
   const int64_t r0 = (int64_t)blockIdx.x * 64 + w * 16;
   const int li = lane & 15, kq = lane >> 4;
   const int64_t ra = r0 + li;
-  const bool rok = ra < D;
+  const double* pa = A + (ra < D ? ra : D - 1);
+  const double* pg[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int j = t * 16 + li;
+    pg[t] = G + (int64_t)(j < n ? j : n - 1) * ldg;
+  }
   d4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = (d4){0.0, 0.0, 0.0, 0.0};
-  for (int k = 0; k < m; k += 4) {
-    const int ka = k + kq;
-    const double a = (rok && ka < m) ? A[ra + (int64_t)ka * lda] : 0.0;
+  constexpr int U = 4;  // steps of 4 columns whose loads travel together
+  int k = 0;
+  for (; k + 4 * U <= m; k += 4 * U) {
+    double a[U], g[U][NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int j = t * 16 + li;
-      const double g = (ka < m && j < n) ? G[ka + (int64_t)j * ldg] : 0.0;
-      acc[t] = mfma16x16x4(a, g, acc[t]);
+    for (int u = 0; u < U; ++u) {
+      const int ka = k + 4 * u + kq;
+      a[u] = pa[(int64_t)ka * lda];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) g[u][t] = pg[t][ka];
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = mfma16x16x4(a[u], g[u][t], acc[t]);
+  }
+  for (; k < m; k += 4) {
+    const int ka = k + kq;
+    const bool kok = ka < m;
+    const int kc = kok ? ka : m - 1;
+    const double a = kok ? pa[(int64_t)kc * lda] : 0.0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = mfma16x16x4(a, pg[t][kc], acc[t]);
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t) {

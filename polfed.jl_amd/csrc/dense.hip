@@ -116,11 +116,11 @@ __global__ __launch_bounds__(256) void gram_kernel(const double* __restrict__ A,
   }
 }
 
-// G[i + j*ldg] = sum_rb part[(rb*m + i)*n + j]  (fixed order: deterministic).
-// The partials are loaded 16 at a time before they are added in order, so
-// one memory round trip serves 16 terms.
+// G[i*gsi + j*gsj] = sum_rb part[(rb*m + i)*n + j]  (fixed order:
+// deterministic).  The partials are loaded 16 at a time before they are added
+// in order, so one memory round trip serves 16 terms.
 __global__ void gram_reduce_kernel(const double* __restrict__ part, int nrb, int m, int n,
-                                   double* __restrict__ G, int ldg) {
+                                   double* __restrict__ G, int64_t gsi, int64_t gsj) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)m * n) return;
   const int i = (int)(idx / n), j = (int)(idx % n);
@@ -137,19 +137,21 @@ __global__ void gram_reduce_kernel(const double* __restrict__ part, int nrb, int
     for (int u = 0; u < U; ++u) t += v[u];
   }
   for (; rb < nrb; ++rb, p += stride) t += *p;
-  G[i + (int64_t)j * ldg] = t;
+  G[i * gsi + j * gsj] = t;
 }
 
 // B = beta*B + alpha*A*G.  Block: 4 waves, 64 rows (wave w: rows r0+16w..+15),
 // all n output columns (NT tiles of 16).  Contraction over the m columns of A
 // in steps of 4: A fragment lane l -> A[r0 + l%16, k + l/16] (16 consecutive
-// rows of one column), G fragment lane l -> G[k + l/16, 16t + l%16].  Four
-// steps' loads are issued before their MFMAs.  Lanes past D or n read a
-// clamped row or column (their results are not stored); only the contraction
-// slots past m in the last steps are zeroed.
+// rows of one column), G fragment lane l -> G(k + l/16, 16t + l%16), element
+// (k, j) at G[k*gsk + j*gsj] (row-major G, gsj = 1, gives the 16 lanes of a
+// k-slot one 128-byte line).  Four steps' loads are issued before their
+// MFMAs.  Lanes past D or n read a clamped row or column (their results are
+// not stored); only the contraction slots past m in the last steps are zeroed.
 template <int NT>
 __global__ __launch_bounds__(256) void update_kernel(const double* __restrict__ A, int64_t lda,
-                                                     int m, const double* __restrict__ G, int ldg,
+                                                     int m, const double* __restrict__ G,
+                                                     int64_t gsk, int64_t gsj,
                                                      double* __restrict__ B, int64_t ldb, int n,
                                                      int64_t D, double alpha, double beta) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(256) void update_kernel(const double* __restrict__ 
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int j = t * 16 + li;
-    pg[t] = G + (int64_t)(j < n ? j : n - 1) * ldg;
+    pg[t] = G + (int64_t)(j < n ? j : n - 1) * gsj;
   }
   d4 acc[NT];
 #pragma unroll
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(256) void update_kernel(const double* __restrict__ 
       const int ka = k + 4 * u + kq;
       a[u] = pa[(int64_t)ka * lda];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) g[u][t] = pg[t][ka];
+      for (int t = 0; t < NT; ++t) g[u][t] = pg[t][ka * gsk];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(256) void update_kernel(const double* __restrict__ 
     const int kc = kok ? ka : m - 1;
     const double a = kok ? pa[(int64_t)kc * lda] : 0.0;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = mfma16x16x4(a, pg[t][kc], acc[t]);
+    for (int t = 0; t < NT; ++t) acc[t] = mfma16x16x4(a, pg[t][kc * gsk], acc[t]);
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
@@ -257,10 +259,13 @@ __global__ void copy_block_kernel(const double* __restrict__ src, int64_t lds, d
 }
 
 // ---------------------------------------------------------------- drivers
+// G(i, j) = <A_i, B_j> stored at G[i*gsi + j*gsj] (column-major: gsi = 1,
+// gsj = ldg >= m; row-major: gsi = ldg >= n, gsj = 1).
 void gram(pf_handle* h, const double* A, int64_t lda, int m, const double* B, int64_t ldb, int n,
-          int64_t D, double* G, int ldg, hipStream_t st) {
+          int64_t D, double* G, int64_t gsi, int64_t gsj, hipStream_t st) {
   PF_REQUIRE(m >= 1 && n >= 1 && n <= kMaxN, "gram: 1 <= n <= 64, m >= 1");
-  PF_REQUIRE(D >= 1 && lda >= D && ldb >= D && ldg >= m, "gram: bad leading dimensions");
+  PF_REQUIRE(D >= 1 && lda >= D && ldb >= D, "gram: bad leading dimensions");
+  PF_REQUIRE((gsi == 1 && gsj >= m) || (gsj == 1 && gsi >= n), "gram: bad output layout");
   const int nrb = (int)std::min<int64_t>(kGramRowBlocks, (D + 15) / 16);
   double* part = reinterpret_cast<double*>(h->partials.get((size_t)nrb * m * n * sizeof(double)));
   dim3 grid(nrb, (m + 63) / 64), block(256);
@@ -274,22 +279,29 @@ void gram(pf_handle* h, const double* A, int64_t lda, int m, const double* B, in
   PF_HIP(hipGetLastError());
   const int64_t mn = (int64_t)m * n;
   hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, part,
-                     nrb, m, n, G, ldg);
+                     nrb, m, n, G, gsi, gsj);
   PF_HIP(hipGetLastError());
 }
 
-void update(const double* A, int64_t lda, int m, const double* G, int ldg, double* B, int64_t ldb,
-            int n, int64_t D, double alpha, double beta, hipStream_t st) {
+// B = beta*B + alpha*A*G with G(k, j) at G[k*gsk + j*gsj] (column-major:
+// gsk = 1, gsj = ldg >= m; row-major: gsk = ldg >= n, gsj = 1).
+void update(const double* A, int64_t lda, int m, const double* G, int64_t gsk, int64_t gsj,
+            double* B, int64_t ldb, int n, int64_t D, double alpha, double beta, hipStream_t st) {
   PF_REQUIRE(m >= 1 && n >= 1 && n <= kMaxN, "update: 1 <= n <= 64, m >= 1");
-  PF_REQUIRE(D >= 1 && lda >= D && ldb >= D && ldg >= m, "update: bad leading dimensions");
+  PF_REQUIRE(D >= 1 && lda >= D && ldb >= D, "update: bad leading dimensions");
+  PF_REQUIRE((gsk == 1 && gsj >= m) || (gsj == 1 && gsk >= n), "update: bad G layout");
   dim3 grid((unsigned)((D + 63) / 64)), block(256);
   const int NT = (n + 15) / 16;
+#define PF_UPDATE(NTV)                                                                         \
+  hipLaunchKernelGGL(update_kernel<NTV>, grid, block, 0, st, A, lda, m, G, gsk, gsj, B, ldb, n, \
+                     D, alpha, beta)
   switch (NT) {
-    case 1: hipLaunchKernelGGL(update_kernel<1>, grid, block, 0, st, A, lda, m, G, ldg, B, ldb, n, D, alpha, beta); break;
-    case 2: hipLaunchKernelGGL(update_kernel<2>, grid, block, 0, st, A, lda, m, G, ldg, B, ldb, n, D, alpha, beta); break;
-    case 3: hipLaunchKernelGGL(update_kernel<3>, grid, block, 0, st, A, lda, m, G, ldg, B, ldb, n, D, alpha, beta); break;
-    default: hipLaunchKernelGGL(update_kernel<4>, grid, block, 0, st, A, lda, m, G, ldg, B, ldb, n, D, alpha, beta); break;
+    case 1: PF_UPDATE(1); break;
+    case 2: PF_UPDATE(2); break;
+    case 3: PF_UPDATE(3); break;
+    default: PF_UPDATE(4); break;
   }
+#undef PF_UPDATE
   PF_HIP(hipGetLastError());
 }
 
@@ -364,7 +376,8 @@ extern "C" int pf_gram(pf_handle* h, const double* A, int64_t lda, int32_t m, co
                        int64_t ldb, int32_t n, int64_t D, double* G, int32_t ldg, pf_stream stream) {
   return guarded([&] {
     PF_REQUIRE(h && A && B && G, "gram: null argument");
-    gram(h, A, lda, m, B, ldb, n, D, G, ldg, S(stream));
+    PF_REQUIRE(ldg >= m, "gram: ldg < m");
+    gram(h, A, lda, m, B, ldb, n, D, G, 1, ldg, S(stream));
   });
 }
 
@@ -374,7 +387,8 @@ extern "C" int pf_gemm_update(pf_handle* h, const double* A, int64_t lda, int32_
   return guarded([&] {
     PF_REQUIRE(h && A && G && B, "update: null argument");
     PF_REQUIRE((const void*)A != (const void*)B, "update: B must not alias A");
-    update(A, lda, m, G, ldg, B, ldb, n, D, alpha, beta, S(stream));
+    PF_REQUIRE(ldg >= m, "update: ldg < m");
+    update(A, lda, m, G, 1, ldg, B, ldb, n, D, alpha, beta, S(stream));
   });
 }
 
@@ -385,10 +399,12 @@ extern "C" int pf_reorth(pf_handle* h, const double* V, int64_t ldv, int32_t k, 
     PF_REQUIRE(passes >= 1 && passes <= 4, "reorth: 1 <= passes <= 4");
     if (k <= 0) return;
     hipStream_t st = S(stream);
+    // G = V^T W held row-major (G(i, j) at G[i*s + j]): the update's 16 lanes
+    // of one contraction index then read one contiguous line of G
     double* G = reinterpret_cast<double*>(h->vecs.get((size_t)k * s * sizeof(double)));
     for (int p = 0; p < passes; ++p) {
-      gram(h, V, ldv, k, W, ldw, s, D, G, k, st);
-      update(V, ldv, k, G, k, W, ldw, s, D, -1.0, 1.0, st);
+      gram(h, V, ldv, k, W, ldw, s, D, G, s, 1, st);
+      update(V, ldv, k, G, s, 1, W, ldw, s, D, -1.0, 1.0, st);
     }
   });
 }
@@ -408,7 +424,7 @@ extern "C" int pf_block_qr(pf_handle* h, double* W, int64_t ldw, int64_t D, int3
     PF_HIP(hipGetLastError());
     std::vector<double> G((size_t)s * s), R1, R2, Ri, R((size_t)s * s, 0.0);
     for (int pass = 0; pass < 2; ++pass) {
-      gram(h, W, ldw, s, W, ldw, s, D, dG, s, st);
+      gram(h, W, ldw, s, W, ldw, s, D, dG, 1, s, st);
       PF_HIP(hipMemcpyAsync(G.data(), dG, G.size() * sizeof(double), hipMemcpyDeviceToHost, st));
       PF_HIP(hipStreamSynchronize(st));
       std::vector<double>& Rp = pass == 0 ? R1 : R2;

@@ -30,90 +30,89 @@ constexpr int kGramRowBlocks = 128;  // row partitions of the Gram reduction
 constexpr int kMaxN = 64;            // widest B block of the Gram/update kernels
 
 // Gram partials: part[(rb * m + i) * n + j] = sum over rb's slabs of A[:, i] B[:, j].
-// Grid: (kGramRowBlocks, ceil(m/64)); block: 4 waves, wave w owns A columns
-// i0 = 64*blockIdx.y + 16*w .. +15 and all n columns of B (NT <= 4 tiles).
-// Slab sl (16 rows) belongs to row block sl mod gridDim.x; a block takes its
-// full slabs two at a time with every load issued before the first MFMA.
-// Lanes past m or n read a clamped column (their results are not stored);
-// only the rows past D of the last, partial slab are zeroed.
-template <int NT>
+// Grid: (kGramRowBlocks, ceil(m/(64*AT))); block: 4 waves, wave w owns the AT
+// tiles of 16 A columns from i0 = 64*AT*blockIdx.y + 16*AT*w and all n columns
+// of B (NT <= 4 tiles), so each B load serves AT A tiles.  Slab sl (16 rows)
+// belongs to row block sl mod gridDim.x; a block takes its full slabs two at
+// a time with every load issued before the first MFMA.  Lanes past m or n
+// read a clamped column (their results are not stored); only the rows past D
+// of the last, partial slab are zeroed.
+template <int NT, int AT>
 __global__ __launch_bounds__(256) void gram_kernel(const double* __restrict__ A, int64_t lda, int m,
                                                    const double* __restrict__ B, int64_t ldb, int n,
                                                    int64_t D, double* __restrict__ part) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int i0 = blockIdx.y * 64 + w * 16;
+  const int i0 = (blockIdx.y * 64 + w * 16) * AT;
   const int li = lane & 15, kq = lane >> 4;
-  const int ia = i0 + li;
-  const double* pa = A + (int64_t)(ia < m ? ia : m - 1) * lda;
+  const double* pa[AT];
+#pragma unroll
+  for (int a = 0; a < AT; ++a) {
+    const int ia = i0 + 16 * a + li;
+    pa[a] = A + (int64_t)(ia < m ? ia : m - 1) * lda;
+  }
   const double* pb[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int jb = t * 16 + li;
     pb[t] = B + (int64_t)(jb < n ? jb : n - 1) * ldb;
   }
-  d4 acc[NT];
+  d4 acc[AT][NT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = (d4){0.0, 0.0, 0.0, 0.0};
+  for (int a = 0; a < AT; ++a)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[a][t] = (d4){0.0, 0.0, 0.0, 0.0};
+  // one slab's loads (rows r0..r0+3 of this lane) and MFMAs; zero rows >= D
+  // only when `tail`
+  auto load = [&](int64_t r0, bool tail, double (&av)[AT][4], double (&bv)[NT][4]) {
+#pragma unroll
+    for (int a = 0; a < AT; ++a)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) av[a][q] = (!tail || r0 + q < D) ? pa[a][r0 + q] : 0.0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bv[t][q] = (!tail || r0 + q < D) ? pb[t][r0 + q] : 0.0;
+  };
+  auto fma = [&](const double (&av)[AT][4], const double (&bv)[NT][4]) {
+#pragma unroll
+    for (int a = 0; a < AT; ++a)
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[a][t] = mfma16x16x4(av[a][q], bv[t][q], acc[a][t]);
+  };
   const int64_t gx = gridDim.x, nfull = D / 16;
   int64_t sl = blockIdx.x;
   for (; sl + gx < nfull; sl += 2 * gx) {
-    double av[2][4], bv[2][NT][4];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int64_t r0 = (sl + h * gx) * 16 + 4 * kq;  // this lane's 4 rows
-#pragma unroll
-      for (int q = 0; q < 4; ++q) av[h][q] = pa[r0 + q];
-#pragma unroll
-      for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) bv[h][t][q] = pb[t][r0 + q];
-    }
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[t] = mfma16x16x4(av[h][q], bv[h][t][q], acc[t]);
+    double av0[AT][4], bv0[NT][4], av1[AT][4], bv1[NT][4];
+    load(sl * 16 + 4 * kq, false, av0, bv0);
+    load((sl + gx) * 16 + 4 * kq, false, av1, bv1);
+    fma(av0, bv0);
+    fma(av1, bv1);
   }
   if (sl < nfull) {  // one full slab left
-    const int64_t r0 = sl * 16 + 4 * kq;
-    double av[4], bv[NT][4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) av[q] = pa[r0 + q];
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) bv[t][q] = pb[t][r0 + q];
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) acc[t] = mfma16x16x4(av[q], bv[t][q], acc[t]);
+    double av[AT][4], bv[NT][4];
+    load(sl * 16 + 4 * kq, false, av, bv);
+    fma(av, bv);
   }
   if (D % 16 != 0 && blockIdx.x == nfull % gx) {  // the partial last slab
-    const int64_t r0 = nfull * 16 + 4 * kq;
-    double av[4], bv[NT][4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) av[q] = (r0 + q < D) ? pa[r0 + q] : 0.0;
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) bv[t][q] = (r0 + q < D) ? pb[t][r0 + q] : 0.0;
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) acc[t] = mfma16x16x4(av[q], bv[t][q], acc[t]);
+    double av[AT][4], bv[NT][4];
+    load(nfull * 16 + 4 * kq, true, av, bv);
+    fma(av, bv);
   }
-  // D-fragment: row (A column) i0 + kq + 4*v, column (B column) 16*t + li
+  // D-fragment: row (A column) i0 + 16a + kq + 4*v, column (B column) 16*t + li
   double* out = part + (int64_t)blockIdx.x * m * n;
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int j = t * 16 + li;
+  for (int a = 0; a < AT; ++a)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = i0 + kq + 4 * v;
-      if (i < m && j < n) out[(int64_t)i * n + j] = acc[t][v];
+    for (int t = 0; t < NT; ++t) {
+      const int j = t * 16 + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + 16 * a + kq + 4 * v;
+        if (i < m && j < n) out[(int64_t)i * n + j] = acc[a][t][v];
+      }
     }
-  }
 }
 
 // G[i*gsi + j*gsj] = sum_rb part[(rb*m + i)*n + j]  (fixed order:
@@ -268,14 +267,28 @@ void gram(pf_handle* h, const double* A, int64_t lda, int m, const double* B, in
   PF_REQUIRE((gsi == 1 && gsj >= m) || (gsj == 1 && gsi >= n), "gram: bad output layout");
   const int nrb = (int)std::min<int64_t>(kGramRowBlocks, (D + 15) / 16);
   double* part = reinterpret_cast<double*>(h->partials.get((size_t)nrb * m * n * sizeof(double)));
-  dim3 grid(nrb, (m + 63) / 64), block(256);
+  // two A tiles per wave once A is wider than one block's 64 columns
+  const int AT = m > 64 ? 2 : 1;
+  dim3 grid(nrb, (m + 64 * AT - 1) / (64 * AT)), block(256);
   const int NT = (n + 15) / 16;
-  switch (NT) {
-    case 1: hipLaunchKernelGGL(gram_kernel<1>, grid, block, 0, st, A, lda, m, B, ldb, n, D, part); break;
-    case 2: hipLaunchKernelGGL(gram_kernel<2>, grid, block, 0, st, A, lda, m, B, ldb, n, D, part); break;
-    case 3: hipLaunchKernelGGL(gram_kernel<3>, grid, block, 0, st, A, lda, m, B, ldb, n, D, part); break;
-    default: hipLaunchKernelGGL(gram_kernel<4>, grid, block, 0, st, A, lda, m, B, ldb, n, D, part); break;
+#define PF_GRAM(NTV, ATV) \
+  hipLaunchKernelGGL((gram_kernel<NTV, ATV>), grid, block, 0, st, A, lda, m, B, ldb, n, D, part)
+  if (AT == 2) {
+    switch (NT) {
+      case 1: PF_GRAM(1, 2); break;
+      case 2: PF_GRAM(2, 2); break;
+      case 3: PF_GRAM(3, 2); break;
+      default: PF_GRAM(4, 2); break;
+    }
+  } else {
+    switch (NT) {
+      case 1: PF_GRAM(1, 1); break;
+      case 2: PF_GRAM(2, 1); break;
+      case 3: PF_GRAM(3, 1); break;
+      default: PF_GRAM(4, 1); break;
+    }
   }
+#undef PF_GRAM
   PF_HIP(hipGetLastError());
   const int64_t mn = (int64_t)m * n;
   hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, part,

@@ -115,6 +115,17 @@ class DensityOfStates:
         val = np.cos(np.multiply.outer(th, self.n)) @ self.coeffs
         return val / (np.pi * np.sqrt(1.0 - x * x))
 
+    def integral(self, a: float, b: float) -> float:
+        """Number of states in [a, b] (-1 <= a <= b <= 1), in closed form:
+        with x = cos(theta) each term integrates to c_n cos(n theta) / pi over
+        theta, i.e. (1/pi) [c_0 (theta_a - theta_b)
+        + sum_{n>=1} c_n (sin(n theta_a) - sin(n theta_b)) / n]."""
+        ta = math.acos(min(max(a, -1.0), 1.0))
+        tb = math.acos(min(max(b, -1.0), 1.0))
+        n = self.n[1:]
+        rest = float(self.coeffs[1:] @ ((np.sin(n * ta) - np.sin(n * tb)) / n))
+        return (float(self.coeffs[0]) * (ta - tb) + rest) / math.pi
+
 
 # ------------------------------------------------------- filter polynomial
 def _poly_alpha_beta(polytype: str):
@@ -246,20 +257,56 @@ def resolve_target(plan: TransformPlan, a: float, b: float, rho) -> None:
         raise ValueError(f"Unsupported target specification: {s}")
 
 
+def _dirichlet(phi: float, K: int) -> float:
+    """1 + 2 sum_{n=1}^K cos(n phi) = sin((K + 1/2) phi) / sin(phi / 2)."""
+    s = math.sin(phi / 2)
+    if abs(s) < 1e-12:
+        return float(2 * K + 1)
+    return math.sin((K + 0.5) * phi) / s
+
+
 def _filter_fn(plan: TransformPlan, center: float, order: int):
+    """x -> p(x) of the normalized filter centred at ``center`` of this order.
+    With the default coefficients (c_0 = 1, c_n = 2 cos(n theta_t),
+    PolfedDefaults.jl:104) on the Chebyshev basis the series sums in closed
+    form: sum_n c_n cos(n theta) = [Dir_K(theta_t - theta) + Dir_K(theta_t +
+    theta)] / 2 with the Dirichlet kernel Dir_K, so each value costs O(1)
+    rather than O(K) (the order and bound bisections evaluate it ~100 times
+    at K up to ~10^5); other coefficients or bases sum the series."""
+    if (plan.coefficients is PolfedDefaults.coefficients and plan.polynomialtype == "Chebyshev"
+            and -1.0 <= center <= 1.0):
+        tt = math.acos(center)
+        K = int(order)
+        raw = lambda th: 0.5 * (_dirichlet(tt - th, K) + _dirichlet(tt + th, K))
+        scale = plan.normalization / raw(tt)
+
+        def p(x):
+            x = float(x)
+            if abs(x) > 1.0:  # outside [-1, 1]: the generic series
+                c = normalized_coefficients(plan.coefficients, center, order,
+                                            plan.normalization, plan.polynomialtype)
+                return float(series_value(c, x, plan.polynomialtype))
+            return scale * raw(math.acos(x))
+        return p
     c = normalized_coefficients(plan.coefficients, center, order, plan.normalization,
                                 plan.polynomialtype)
     return lambda x: float(series_value(c, x, plan.polynomialtype))
 
 
 def getbounds_from_dos(plan: TransformPlan, rho, position: str = "mean") -> None:
-    """``getbounds_from_dos!`` (GetBounds.jl:57-81)."""
+    """``getbounds_from_dos!`` (GetBounds.jl:57-81).  The state count over the
+    interval is the KPM density's closed-form integral
+    (``DensityOfStates.integral``), the quantity the reference integrates with
+    QuadGK; any other density is integrated numerically (scipy ``quad``)."""
     lo, hi = _BOUNDS[position]
     t = plan.target
 
     def f(d):
         a_, b_ = max(lo(t, d), -1.0), min(hi(t, d), 1.0)
-        n, _ = quad(lambda x: float(rho(x)), a_, b_, epsabs=1e-12, epsrel=1e-12, limit=500)
+        if isinstance(rho, DensityOfStates):
+            n = rho.integral(a_, b_)
+        else:
+            n, _ = quad(lambda x: float(rho(x)), a_, b_, epsabs=1e-12, epsrel=1e-12, limit=500)
         return -1 if n >= plan.howmany else 1
 
     d = bisection(f, 0.0, 1.0, tol=1e-8)

@@ -111,6 +111,41 @@ def test_bounds_and_order_host_mirror_match_oracle():
         spc.getbounds(bad, 1.0, 0.0, rho_h)
 
 
+def test_closed_form_filter_and_state_count_match_the_series():
+    """The O(1) evaluations behind the order and bound bisections: the default
+    Chebyshev filter through the Dirichlet kernel against its K-term series
+    (numpy chebval, an independent summation), the KPM state count against
+    adaptive quadrature, and whole interval/order selections against the
+    series-and-quadrature path (a coefficient function that is not the
+    default object, and a density given as a plain callable)."""
+    from numpy.polynomial import chebyshev as npc
+    for t, K in [(0.0, 12), (0.37, 401), (-0.8, 5000), (0.999, 33)]:
+        plan = spc.build_transform_plan(TransformConfig(), 1.0, 0.0, 1, ("rescaled", t))
+        p = spc._filter_fn(plan, t, K)
+        c = spc.raw_coefficients(PolfedDefaults.coefficients, t, K)
+        c = c / npc.chebval(t, c)
+        for x in (-1.0, -0.5, t - 1e-3, t, 0.25, 1.0):
+            x = min(max(x, -1.0), 1.0)
+            assert abs(p(x) - npc.chebval(x, c)) <= 1e-12 * max(1.0, abs(npc.chebval(x, c)))
+    mu = odos.exact_moments(np.cos(np.linspace(0.1, 3.0, 2000)), 250) * 300.0
+    rho = spc.DensityOfStates(mu, "Jackson")
+    from scipy.integrate import quad
+    for a, b in [(-0.9, -0.2), (-0.01, 0.01), (0.3, 0.99), (-1.0, 1.0)]:
+        q, _ = quad(lambda x: float(rho(x)), a, b, epsabs=1e-12, epsrel=1e-12, limit=500)
+        assert abs(rho.integral(a, b) - q) <= 1e-8 * max(1.0, abs(q))
+    generic = lambda t, n: PolfedDefaults.coefficients(t, n)
+    for target, howmany in [(("rescaled", 0.0), 50), (("rescaled", -0.45), 120), ("maxdos", 20)]:
+        pf = spc.build_transform_plan(TransformConfig(), 1.0, 0.0, howmany, target)
+        spc.getbounds(pf, 1.0, 0.0, rho)
+        spc.getorderofexpansion(pf)
+        pg = spc.build_transform_plan(TransformConfig(coefficients=generic), 1.0, 0.0, howmany,
+                                      target)
+        spc.getbounds(pg, 1.0, 0.0, lambda x: rho(x))
+        spc.getorderofexpansion(pg)
+        assert pf.order == pg.order, (target, pf.order, pg.order)
+        assert abs(pf.left - pg.left) < 1e-9 and abs(pf.right - pg.right) < 1e-9
+
+
 def test_defaults_and_configs():
     for howmany, s, eta, exp in KAT["expectedkrylovdim"]["cases"]:
         assert PolfedDefaults.expectedkrylovdim(howmany, s, eta) == exp
